@@ -1,0 +1,964 @@
+// ggbtd.hip -- gfx950 kernels and C ABI of the real-time TDDFT propagator.
+//
+// One time step (t_n -> t_n + dt) is the palindrome
+//     Vh . Le . Lo . K . Lo . Le . Vh         (right-most acts first)
+// on psi[l][c][i] (complex, u-units); see include/ggbtd.h for layouts.
+//   Vh     theta-space phase: Psi = B psi, Psi *= mask e^{-i dV dt/2}, psi = B^T Psi
+//   Le/Lo  laser r E cos(theta) for dt/2 as 2x2 rotations on (l, l+1) blocks
+//   K      per-l exp(-i H_l dt) = Q diag(e^{-i lam dt}) Q^T on the retained
+//          eigenpairs (the reference's per-l cublasZgemmBatched,
+//          h_timePropagation.cuh:17-26)
+// followed by the potential rebuild from psi_{n+1} (explicit update as in
+// h_timePropagation.cuh:97-101) and the observables (dipoleMoment.cuh,
+// dipoleAcceleration.cuh) as fused reductions.
+//
+// Kernels, launch order per step:
+//   k_angular<mode 0>   Vh, Le, Lo                 grid (ceil(N/8), C)
+//   k_kin_fwd           Y = e^{-i lam dt} Q^T psi   grid (nk/64, 2C/16, n_l)
+//   k_kin_bwd           psi = Q Y                   grid (N/64,  2C/16, n_l)
+//   k_angular<mode 1>   Lo, Le, Vh, |B psi|^2, observable partials
+//   k_obs_reduce        per-column observable sums
+//   k_density           rho_u, v_xc, multipole moments q_L   grid (ceil(N/8), n_inst)
+//   k_hartree           V_L by prefix/suffix affine scans   grid (n_har, n_inst)
+//   k_dv                dV and the phase factors of the next step
+//
+// All matrix products run on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+// Lane layout of that instruction (checked by ggb_selftest_mfma):
+//   A (16x4):  lane L holds A[L % 16][L / 16]
+//   B (4x16):  lane L holds B[L / 16][L % 16]
+//   D (16x16): lane L holds D[4 (L / 16) + r][L % 16], r = 0..3
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "ggbtd.h"
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+#define WG 256
+#define RT 8          // radial nodes per angular tile (16 real columns)
+#define RHO_MIN 1e-40 // v_xc = 0 at or below this density
+
+static __device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+static __device__ __forceinline__ double c_lm(int l, int m) {
+    // <l m|cos theta|l+1 m> (internal.py:80-84, angularCoefficients.cuh d_c)
+    double dl = (double)l, dm = (double)m;
+    return sqrt(((dl + 1.0) * (dl + 1.0) - dm * dm) / ((2.0 * dl + 1.0) * (2.0 * dl + 3.0)));
+}
+
+// ---------------------------------------------------------------------------
+// Kinetic propagator, pass 1: Y[l][c][a] = e^{-i lam_a dt} sum_i Q[l][i][a] psi[l][c][i]
+// M = eigen index a, N = real column (c, re/im), K = radial node i.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_kin_fwd(const double *__restrict__ Q,
+                                                const double *__restrict__ lam,
+                                                const int32_t *__restrict__ nkeep,
+                                                const double *__restrict__ psi,
+                                                double *__restrict__ Y, int N, int nk_max, int C,
+                                                double dt) {
+    const int l = blockIdx.z;
+    const int nk = nkeep[l];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a0 = (blockIdx.x * 4 + wave) * 16;
+    if (a0 >= nk) return;  // wave-uniform
+    const int kr = lane >> 4, lc = lane & 15;
+    const int col = blockIdx.y * 16 + lc;
+    const int c = col >> 1, p = col & 1;
+    const bool colok = c < C;
+    const double *Ql = Q + (size_t)l * N * nk_max;
+    const double *Xl = psi + (size_t)l * C * N * 2;
+    const int am = a0 + lc;
+    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+    for (int k0 = 0; k0 < N; k0 += 4) {
+        const int i = k0 + kr;
+        const double av = (i < N) ? Ql[(size_t)i * nk_max + am] : 0.0;
+        const double bv = (colok && i < N) ? Xl[((size_t)c * N + i) * 2 + p] : 0.0;
+        acc = mfma64(av, bv, acc);
+    }
+    const double *laml = lam + (size_t)l * nk_max;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int a = a0 + 4 * kr + r;
+        const double v = acc[r];
+        const double vo = __shfl_xor(v, 1);
+        const double re = p ? vo : v, im = p ? v : vo;
+        double sn, cs;
+        sincos(laml[a < nk ? a : 0] * dt, &sn, &cs);
+        const double o = p ? (cs * im - sn * re) : (cs * re + sn * im);
+        if (colok && a < nk) Y[(((size_t)l * C + c) * nk_max + a) * 2 + p] = o;
+    }
+}
+
+// Kinetic propagator, pass 2: psi[l][c][i] = sum_a Q[l][i][a] Y[l][c][a]
+__global__ __launch_bounds__(WG) void k_kin_bwd(const double *__restrict__ Q,
+                                                const int32_t *__restrict__ nkeep,
+                                                const double *__restrict__ Y,
+                                                double *__restrict__ psi, int N, int nk_max,
+                                                int C) {
+    const int l = blockIdx.z;
+    const int nk = nkeep[l];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i0 = (blockIdx.x * 4 + wave) * 16;
+    if (i0 >= N) return;  // wave-uniform
+    const int kr = lane >> 4, lc = lane & 15;
+    const int col = blockIdx.y * 16 + lc;
+    const int c = col >> 1, p = col & 1;
+    const bool colok = c < C;
+    const double *Ql = Q + (size_t)l * N * nk_max;
+    const double *Yl = Y + (size_t)l * C * nk_max * 2;
+    const int im_ = i0 + lc;
+    const bool rowok = im_ < N;
+    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+    for (int k0 = 0; k0 < nk; k0 += 4) {
+        const int a = k0 + kr;
+        const double av = (rowok && a < nk) ? Ql[(size_t)im_ * nk_max + a] : 0.0;
+        const double bv = (colok && a < nk) ? Yl[((size_t)c * nk_max + a) * 2 + p] : 0.0;
+        acc = mfma64(av, bv, acc);
+    }
+    double *Xl = psi + (size_t)l * C * N * 2;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = i0 + 4 * kr + r;
+        if (colok && i < N) Xl[((size_t)c * N + i) * 2 + p] = acc[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Angular kernel.  One work-group = one orbital column c, RT radial nodes.
+// LDS holds the l-space tile X[Nlp][16] and the theta-space tile P[Nthp][16]
+// (real column n = 2 * i_local + re/im).
+// mode 0: Vh, Le, Lo          (first half of the step)
+// mode 1: Lo, Le, Vh, store, then |B psi|^2 + observable partials
+// mode 2: |B psi|^2 + observable partials only (initial potential)
+// ---------------------------------------------------------------------------
+struct AngArgs {
+    double *psi;
+    const double *B;       // [n_m][Nthp][Nlp] zero padded
+    const int32_t *mcol;   // [n_orb]
+    const double2 *phase;  // [n_inst][N][Nth] mask e^{-i dV dt/2}
+    const double *r;
+    const double *chi;     // [Nth]
+    const double *E;       // [n_inst][n_field]
+    double *dens;          // [C][N][Nth] |Psi|^2
+    double *obs_part;      // [C][ntiles][3]
+    int N, Nl, Nlp, Nth, Nthp, C, n_orb, n_field, step;
+    double dt;
+};
+
+static __device__ void ang_laser(double *X, int parity, int m, int Nl, const double *a_i) {
+    // pairs (l, l+1), l = m + parity + 2 q, l + 1 < Nl; thread -> (q, i_local)
+    const int t = threadIdx.x;
+    const int il = t & 7;
+    for (int q = t >> 3;; q += WG / 8) {
+        const int l = m + parity + 2 * q;
+        if (l + 1 >= Nl) break;
+        const double th = a_i[il] * c_lm(l, m);
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        double *x0 = X + l * 16 + 2 * il;
+        double *x1 = X + (l + 1) * 16 + 2 * il;
+        const double r0 = x0[0], i0 = x0[1], r1 = x1[0], i1 = x1[1];
+        // [x0', x1'] = [[cs, -i sn], [-i sn, cs]] [x0, x1]
+        x0[0] = cs * r0 + sn * i1;
+        x0[1] = cs * i0 - sn * r1;
+        x1[0] = cs * r1 + sn * i0;
+        x1[1] = cs * i1 - sn * r0;
+    }
+}
+
+// theta-space product P = B X; accumulators for up to 4 M-tiles per wave.
+static __device__ void ang_fwd(const double *Bm, const double *X, int Nlp, int Nthp, d4_t *acc) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int mt = Nthp / 16;
+    for (int t = 0; t < 4; ++t) {
+        acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        const int tile = wave + 4 * t;
+        if (tile >= mt) continue;
+        const double *Arow = Bm + (size_t)(tile * 16 + lc) * Nlp;
+        d4_t a = acc[t];
+        for (int k0 = 0; k0 < Nlp; k0 += 4) {
+            a = mfma64(Arow[k0 + kr], X[(k0 + kr) * 16 + lc], a);
+        }
+        acc[t] = a;
+    }
+}
+
+// l-space product X = B^T P; up to 2 M-tiles per wave.
+static __device__ void ang_bwd(const double *Bm, const double *P, int Nlp, int Nthp, d4_t *acc) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int mt = Nlp / 16;
+    for (int t = 0; t < 2; ++t) {
+        acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        const int tile = wave + 4 * t;
+        if (tile >= mt) continue;
+        d4_t a = acc[t];
+        for (int k0 = 0; k0 < Nthp; k0 += 4) {
+            a = mfma64(Bm[(size_t)(k0 + kr) * Nlp + tile * 16 + lc], P[(k0 + kr) * 16 + lc], a);
+        }
+        acc[t] = a;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
+    extern __shared__ double lds[];
+    const int Nlp = g.Nlp, Nthp = g.Nthp;
+    double *X = lds;                // [Nlp][16]
+    double *P = lds + Nlp * 16;     // [Nthp][16]
+    __shared__ double a_i[RT];
+    __shared__ double red[3][WG / 64];
+
+    const int tile_i = blockIdx.x, c = blockIdx.y;
+    const int i0 = tile_i * RT;
+    const int inst = c / g.n_orb;
+    const int m = g.mcol[c % g.n_orb];
+    const double *Bm = g.B + (size_t)m * Nthp * Nlp;
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int ilc = lc >> 1, p = lc & 1;  // column -> (i_local, re/im)
+
+    // load the l-space tile
+    for (int idx = t; idx < Nlp * RT; idx += WG) {
+        const int l = idx / RT, il = idx % RT;
+        const int i = i0 + il;
+        double2 v = make_double2(0.0, 0.0);
+        if (l < g.Nl && i < g.N) v = ((const double2 *)g.psi)[((size_t)l * g.C + c) * g.N + i];
+        X[l * 16 + 2 * il] = v.x;
+        X[l * 16 + 2 * il + 1] = v.y;
+    }
+    if (MODE != 2 && t < RT) {
+        const int i = i0 + t;
+        const double Ev = g.E[(size_t)inst * g.n_field + g.step];
+        a_i[t] = (i < g.N) ? 0.5 * g.dt * Ev * g.r[i] : 0.0;
+    }
+    __syncthreads();
+
+    if (MODE == 1) {
+        ang_laser(X, 1, m, g.Nl, a_i);
+        __syncthreads();
+        ang_laser(X, 0, m, g.Nl, a_i);
+        __syncthreads();
+    }
+
+    d4_t acc[4];
+    if (MODE != 2) {
+        // Vh: theta space, phase, back to l space
+        ang_fwd(Bm, X, Nlp, Nthp, acc);
+        const double2 *ph = g.phase + (size_t)inst * g.N * g.Nth;
+        const int i = i0 + ilc;
+        for (int tt = 0; tt < 4; ++tt) {
+            const int tile = wave + 4 * tt;
+            if (tile >= Nthp / 16) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = tile * 16 + 4 * kr + r;
+                const double v = acc[tt][r];
+                const double vo = __shfl_xor(v, 1);
+                double o = 0.0;
+                if (j < g.Nth && i < g.N) {
+                    const double2 f = ph[(size_t)i * g.Nth + j];
+                    const double re = p ? vo : v, im = p ? v : vo;
+                    o = p ? (re * f.y + im * f.x) : (re * f.x - im * f.y);
+                }
+                P[j * 16 + lc] = o;
+            }
+        }
+        __syncthreads();
+        ang_bwd(Bm, P, Nlp, Nthp, acc);
+        for (int tt = 0; tt < 2; ++tt) {
+            const int tile = wave + 4 * tt;
+            if (tile >= Nlp / 16) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) X[(tile * 16 + 4 * kr + r) * 16 + lc] = acc[tt][r];
+        }
+        __syncthreads();
+    }
+
+    if (MODE == 0) {
+        ang_laser(X, 0, m, g.Nl, a_i);
+        __syncthreads();
+        ang_laser(X, 1, m, g.Nl, a_i);
+        __syncthreads();
+    }
+
+    if (MODE != 2) {
+        for (int idx = t; idx < g.Nl * RT; idx += WG) {
+            const int l = idx / RT, il = idx % RT;
+            const int i = i0 + il;
+            if (i < g.N)
+                ((double2 *)g.psi)[((size_t)l * g.C + c) * g.N + i] =
+                    make_double2(X[l * 16 + 2 * il], X[l * 16 + 2 * il + 1]);
+        }
+    }
+    if (MODE == 0) return;
+
+    // |B psi|^2 and observable partials
+    ang_fwd(Bm, X, Nlp, Nthp, acc);
+    const int i = i0 + ilc;
+    const bool iok = i < g.N;
+    const double ri = iok ? g.r[i] : 1.0;
+    double sd = 0.0, sa = 0.0, sn = 0.0;
+    double *dens = g.dens + (size_t)c * g.N * g.Nth;
+    for (int tt = 0; tt < 4; ++tt) {
+        const int tile = wave + 4 * tt;
+        if (tile >= Nthp / 16) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = tile * 16 + 4 * kr + r;
+            const double v = acc[tt][r];
+            const double s2 = v * v + __shfl_xor(v * v, 1);
+            if (p == 0 && iok && j < g.Nth) {
+                dens[(size_t)i * g.Nth + j] = s2;
+                const double cj = g.chi[j];
+                sd += ri * cj * s2;
+                sa += cj * s2 / (ri * ri);
+                sn += s2;
+            }
+        }
+    }
+    // work-group reduction, fixed order
+    for (int off = 32; off > 0; off >>= 1) {
+        sd += __shfl_down(sd, off);
+        sa += __shfl_down(sa, off);
+        sn += __shfl_down(sn, off);
+    }
+    if (lane == 0) {
+        red[0][wave] = sd;
+        red[1][wave] = sa;
+        red[2][wave] = sn;
+    }
+    __syncthreads();
+    if (t < 3) {
+        double s = 0.0;
+        for (int w = 0; w < WG / 64; ++w) s += red[t][w];
+        g.obs_part[((size_t)c * gridDim.x + tile_i) * 3 + t] = s;
+    }
+}
+
+// per-column sums of the observable partials -> obs[step_rel][c][3]
+__global__ void k_obs_reduce(const double *__restrict__ obs_part, double *__restrict__ obs,
+                             int C, int ntiles) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= C * 3) return;
+    const int c = idx / 3, k = idx % 3;
+    double s = 0.0;
+    for (int tl = 0; tl < ntiles; ++tl) s += obs_part[((size_t)c * ntiles + tl) * 3 + k];
+    obs[idx] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Density, LDA v_xc and multipole moments for one instance and RT radial nodes.
+// ---------------------------------------------------------------------------
+static __device__ double lda_vxc(double rho) {
+    // Slater exchange + Perdew-Wang 1992 correlation, spin unpolarised
+    if (!(rho > RHO_MIN)) return 0.0;
+    const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+    const double vx = -cbrt(3.0 * rho / M_PI);
+    const double rs = cbrt(3.0 / (4.0 * M_PI * rho));
+    const double srs = sqrt(rs);
+    const double q0 = -2.0 * A * (1.0 + a1 * rs);
+    const double q1 = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+    const double dq1 = A * (b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 4.0 * b4 * rs);
+    const double lg = log1p(1.0 / q1);
+    const double eps = q0 * lg;
+    const double deps = -2.0 * A * a1 * lg - q0 * dq1 / (q1 * q1 + q1);
+    return vx + eps - rs / 3.0 * deps;
+}
+
+__global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
+                                                const double *__restrict__ occ,
+                                                const double *__restrict__ rho_frozen,
+                                                const double *__restrict__ r,
+                                                const double *__restrict__ wr,
+                                                const double *__restrict__ omega,
+                                                const double *__restrict__ PL,
+                                                double *__restrict__ vxc, double *__restrict__ q,
+                                                int N, int Nth, int n_orb, int n_har) {
+    extern __shared__ double rho_s[];  // [RT][Nth]
+    const int i0 = blockIdx.x * RT, inst = blockIdx.y;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < RT * Nth; idx += WG) {
+        const int il = idx / Nth, j = idx % Nth;
+        const int i = i0 + il;
+        double ru = 0.0;
+        if (i < N) {
+            for (int k = 0; k < n_orb; ++k)
+                ru += occ[k] * dens[((size_t)(inst * n_orb + k) * N + i) * Nth + j];
+            if (rho_frozen) ru += 0.5 * rho_frozen[i] * omega[j];
+            const double rho = ru / (2.0 * M_PI * omega[j] * wr[i] * r[i] * r[i]);
+            vxc[((size_t)inst * N + i) * Nth + j] = lda_vxc(rho);
+        }
+        rho_s[idx] = ru;
+    }
+    __syncthreads();
+    for (int idx = t; idx < RT * n_har; idx += WG) {
+        const int il = idx / n_har, L = idx % n_har;
+        const int i = i0 + il;
+        if (i >= N) continue;
+        double s = 0.0;
+        for (int j = 0; j < Nth; ++j) s += PL[(size_t)j * n_har + L] * rho_s[il * Nth + j];
+        q[((size_t)inst * n_har + L) * N + i] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hartree multipole V_L(r_a) = sum_b q_L(b) r_<^L / r_>^{L+1} by two affine
+// scans (ratio form, no overflow):
+//   A_a = A_{a-1} (r_{a-1}/r_a)^{L+1} + q_a / r_a
+//   B_a = (r_a/r_{a+1})^L (B_{a+1} + q_{a+1} / r_{a+1}),   V = A + B
+// One work-group per (L, instance); each thread composes a contiguous chunk,
+// then a Hillis-Steele scan of the chunk maps in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
+                                                const double *__restrict__ r,
+                                                double *__restrict__ V, int N, int n_har) {
+    __shared__ double sa[WG], sb[WG];
+    const int L = blockIdx.x, inst = blockIdx.y;
+    const int t = threadIdx.x;
+    const double *qL = q + ((size_t)inst * n_har + L) * N;
+    double *VL = V + ((size_t)inst * n_har + L) * N;
+    const int cs = (N + WG - 1) / WG;
+    const int s0 = t * cs, s1 = min(N, s0 + cs);
+
+    // ---- prefix: A_a = alpha_a A_{a-1} + beta_a
+    double al = 1.0, be = 0.0;
+    for (int a = s0; a < s1; ++a) {
+        const double alpha = (a > 0) ? pow(r[a - 1] / r[a], (double)(L + 1)) : 0.0;
+        al *= alpha;
+        be = be * alpha + qL[a] / r[a];
+    }
+    sa[t] = al;
+    sb[t] = be;
+    __syncthreads();
+    for (int off = 1; off < WG; off <<= 1) {
+        double pa = 1.0, pb = 0.0;
+        if (t >= off) {
+            pa = sa[t - off];
+            pb = sb[t - off];
+        }
+        __syncthreads();
+        if (t >= off) {
+            // left map (pa, pb) then this map (sa, sb)
+            sb[t] = pb * sa[t] + sb[t];
+            sa[t] = pa * sa[t];
+        }
+        __syncthreads();
+    }
+    double A = (t > 0) ? sb[t - 1] : 0.0;
+    __syncthreads();
+    for (int a = s0; a < s1; ++a) {
+        const double alpha = (a > 0) ? pow(r[a - 1] / r[a], (double)(L + 1)) : 0.0;
+        A = A * alpha + qL[a] / r[a];
+        VL[a] = A;
+    }
+    __syncthreads();
+
+    // ---- suffix over reversed index: B_a = gamma_a B_{a+1} + gamma_a beta_{a+1}
+    // thread t owns the same chunk, scanned from the right
+    al = 1.0;
+    be = 0.0;
+    for (int a = s1 - 1; a >= s0; --a) {
+        if (a == N - 1) {
+            al = 0.0;  // B_{N-1} = 0
+            be = 0.0;
+            continue;
+        }
+        const double gam = pow(r[a] / r[a + 1], (double)L);
+        al *= gam;
+        be = be * gam + gam * qL[a + 1] / r[a + 1];
+    }
+    // reversed scan: thread t combines with threads t' > t
+    sa[WG - 1 - t] = al;
+    sb[WG - 1 - t] = be;
+    __syncthreads();
+    for (int off = 1; off < WG; off <<= 1) {
+        const int u = WG - 1 - t;
+        double pa = 1.0, pb = 0.0;
+        if (u >= off) {
+            pa = sa[u - off];
+            pb = sb[u - off];
+        }
+        __syncthreads();
+        if (u >= off) {
+            sb[u] = pb * sa[u] + sb[u];
+            sa[u] = pa * sa[u];
+        }
+        __syncthreads();
+    }
+    double Bv = (t < WG - 1) ? sb[WG - 1 - t - 1] : 0.0;
+    for (int a = s1 - 1; a >= s0; --a) {
+        if (a == N - 1) {
+            Bv = 0.0;
+        } else {
+            const double gam = pow(r[a] / r[a + 1], (double)L);
+            Bv = gam * (Bv + qL[a + 1] / r[a + 1]);
+        }
+        VL[a] += Bv;
+    }
+}
+
+// dV = sum_L P_L(chi_j) V_L(r_i) + v_xc - v_gs; phase = mask e^{-i dV dt/2}
+__global__ __launch_bounds__(WG) void k_dv(const double *__restrict__ V,
+                                           const double *__restrict__ vxc,
+                                           const double *__restrict__ PL,
+                                           const double *__restrict__ v_gs,
+                                           const double *__restrict__ mask,
+                                           double *__restrict__ dv, double2 *__restrict__ phase,
+                                           int N, int Nth, int n_har, double dt, int interacting) {
+    extern __shared__ double Vt[];  // [n_har][RT]
+    const int i0 = blockIdx.x * RT, inst = blockIdx.y;
+    const int t = threadIdx.x;
+    if (interacting) {
+        for (int idx = t; idx < n_har * RT; idx += WG) {
+            const int L = idx / RT, il = idx % RT;
+            const int i = i0 + il;
+            Vt[idx] = (i < N) ? V[((size_t)inst * n_har + L) * N + i] : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < RT * Nth; idx += WG) {
+        const int il = idx / Nth, j = idx % Nth;
+        const int i = i0 + il;
+        if (i >= N) continue;
+        double s = 0.0;
+        if (interacting) {
+            for (int L = 0; L < n_har; ++L) s += PL[(size_t)j * n_har + L] * Vt[L * RT + il];
+            s += vxc[((size_t)inst * N + i) * Nth + j] - v_gs[i];
+        }
+        const size_t o = ((size_t)inst * N + i) * Nth + j;
+        dv[o] = s;
+        double sn, cs;
+        sincos(0.5 * dt * s, &sn, &cs);
+        phase[o] = make_double2(mask[i] * cs, -mask[i] * sn);
+    }
+}
+
+__global__ void k_selftest_mfma(double *out) {
+    const int lane = threadIdx.x;
+    const double a = (double)(lane % 16) + 0.25 * (double)(lane / 16);
+    const double b = 0.5 * (double)(lane / 16) - (double)(lane % 16);
+    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+    acc = mfma64(a, b, acc);
+    for (int r = 0; r < 4; ++r) out[(4 * (lane / 16) + r) * 16 + lane % 16] = acc[r];
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct ggb_ctx {
+    ggb_problem p;
+    int Nlp, Nthp, C, ntiles;
+    hipStream_t stream;
+    std::string err;
+    bool have_static, have_orbitals;
+    int n_field;
+    // device buffers
+    double *psi, *Y, *Q, *lam, *B, *PL, *chi, *omega, *r, *wr, *v_gs, *mask, *occ, *rho_frozen;
+    int32_t *nkeep, *mcol;
+    double *dens, *vxc, *q, *V, *dv, *obs_part, *obs, *E;
+    double2 *phase;
+    size_t obs_cap;
+};
+
+static std::string g_create_err;
+
+#define HIPCHK(ctx, call)                                                               \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);             \
+            return GGB_E_HIP;                                                           \
+        }                                                                               \
+    } while (0)
+
+static int fail(ggb_ctx *ctx, int code, const std::string &msg) {
+    ctx->err = msg;
+    return code;
+}
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T));
+}
+
+extern "C" int ggb_abi_version(void) { return GGB_ABI_VERSION; }
+
+extern "C" const char *ggb_last_error(const ggb_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+extern "C" void ggb_destroy(ggb_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->p.device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->lam, ctx->B, ctx->PL, ctx->chi, ctx->omega,
+                    ctx->r, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
+                    ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->q, ctx->V, ctx->dv,
+                    ctx->obs_part, ctx->obs, ctx->E, ctx->phase};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
+    if (!p || !out) {
+        g_create_err = "ggb_create: null argument";
+        return GGB_E_ARG;
+    }
+    *out = nullptr;
+    if (p->abi_version != GGB_ABI_VERSION) {
+        g_create_err = "ggb_create: abi_version mismatch";
+        return GGB_E_ARG;
+    }
+    if (p->n_r < 4 || p->n_r > 4096 || p->n_l < 2 || p->n_l > 128 || p->n_theta < p->n_l ||
+        p->n_theta > 256 || p->n_har < 1 || p->n_har > 256 || p->n_orb < 1 || p->n_inst < 1 ||
+        p->n_m < 1 || p->n_m > p->n_l || p->nk_max < 16 || p->nk_max % 16 != 0 ||
+        p->nk_max > ((p->n_r + 15) / 16) * 16 || !(p->dt > 0.0) || p->precision != 0 ||
+        (long)p->n_orb * p->n_inst > 65535) {
+        g_create_err = "ggb_create: problem sizes out of range";
+        return GGB_E_ARG;
+    }
+    ggb_ctx *ctx = new ggb_ctx();
+    memset((void *)&ctx->p, 0, sizeof(ctx->p));
+    ctx->p = *p;
+    ctx->Nlp = ((p->n_l + 15) / 16) * 16;
+    ctx->Nthp = ((p->n_theta + 15) / 16) * 16;
+    ctx->C = p->n_orb * p->n_inst;
+    ctx->ntiles = (p->n_r + RT - 1) / RT;
+    ctx->have_static = ctx->have_orbitals = false;
+    ctx->n_field = 0;
+    ctx->obs_cap = 0;
+    ctx->psi = ctx->Y = ctx->Q = ctx->lam = ctx->B = ctx->PL = ctx->chi = ctx->omega = nullptr;
+    ctx->r = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
+    ctx->nkeep = ctx->mcol = nullptr;
+    ctx->dens = ctx->vxc = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
+    ctx->phase = nullptr;
+    ctx->stream = nullptr;
+
+    auto bail = [&](const std::string &m, int code) {
+        g_create_err = m;
+        ggb_destroy(ctx);
+        return code;
+    };
+    if (hipSetDevice(p->device) != hipSuccess) return bail("ggb_create: hipSetDevice failed", GGB_E_HIP);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail("ggb_create: hipStreamCreate failed", GGB_E_HIP);
+    const size_t N = p->n_r, Nl = p->n_l, C = ctx->C, Nth = p->n_theta, nI = p->n_inst;
+    bool ok = true;
+    ok &= dalloc(&ctx->psi, Nl * C * N * 2) == hipSuccess;
+    ok &= dalloc(&ctx->Y, Nl * C * (size_t)p->nk_max * 2) == hipSuccess;
+    ok &= dalloc(&ctx->Q, Nl * N * (size_t)p->nk_max) == hipSuccess;
+    ok &= dalloc(&ctx->lam, Nl * (size_t)p->nk_max) == hipSuccess;
+    ok &= dalloc(&ctx->nkeep, Nl) == hipSuccess;
+    ok &= dalloc(&ctx->B, (size_t)p->n_m * ctx->Nthp * ctx->Nlp) == hipSuccess;
+    ok &= dalloc(&ctx->PL, Nth * (size_t)p->n_har) == hipSuccess;
+    ok &= dalloc(&ctx->chi, Nth) == hipSuccess;
+    ok &= dalloc(&ctx->omega, Nth) == hipSuccess;
+    ok &= dalloc(&ctx->r, N) == hipSuccess;
+    ok &= dalloc(&ctx->wr, N) == hipSuccess;
+    ok &= dalloc(&ctx->v_gs, N) == hipSuccess;
+    ok &= dalloc(&ctx->mask, N) == hipSuccess;
+    ok &= dalloc(&ctx->occ, (size_t)p->n_orb) == hipSuccess;
+    ok &= dalloc(&ctx->mcol, (size_t)p->n_orb) == hipSuccess;
+    ok &= dalloc(&ctx->dens, C * N * Nth) == hipSuccess;
+    ok &= dalloc(&ctx->vxc, nI * N * Nth) == hipSuccess;
+    ok &= dalloc(&ctx->q, nI * (size_t)p->n_har * N) == hipSuccess;
+    ok &= dalloc(&ctx->V, nI * (size_t)p->n_har * N) == hipSuccess;
+    ok &= dalloc(&ctx->dv, nI * N * Nth) == hipSuccess;
+    ok &= dalloc(&ctx->phase, nI * N * Nth) == hipSuccess;
+    ok &= dalloc(&ctx->obs_part, C * (size_t)ctx->ntiles * 3) == hipSuccess;
+    if (!ok) return bail("ggb_create: device allocation failed", GGB_E_NOMEM);
+    // zero everything that kernels may read before it is written
+    if (hipMemset(ctx->B, 0, (size_t)p->n_m * ctx->Nthp * ctx->Nlp * sizeof(double)) != hipSuccess ||
+        hipMemset(ctx->Q, 0, Nl * N * (size_t)p->nk_max * sizeof(double)) != hipSuccess ||
+        hipMemset(ctx->lam, 0, Nl * (size_t)p->nk_max * sizeof(double)) != hipSuccess ||
+        hipMemset(ctx->Y, 0, Nl * C * (size_t)p->nk_max * 2 * sizeof(double)) != hipSuccess ||
+        hipMemset(ctx->psi, 0, Nl * C * N * 2 * sizeof(double)) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+        return bail("ggb_create: hipMemset failed", GGB_E_HIP);
+    *out = ctx;
+    return GGB_OK;
+}
+
+extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
+    if (!ctx) return GGB_E_ARG;
+    if (!s || !s->r || !s->wr || !s->v_gs || !s->mask || !s->Q || !s->lam || !s->nkeep || !s->B ||
+        !s->PL || !s->chi || !s->omega || !s->m || !s->occ)
+        return fail(ctx, GGB_E_ARG, "ggb_upload_static: null table");
+    const ggb_problem &p = ctx->p;
+    for (int l = 0; l < p.n_l; ++l)
+        if (s->nkeep[l] < 1 || s->nkeep[l] > p.nk_max)
+            return fail(ctx, GGB_E_ARG, "ggb_upload_static: nkeep out of range");
+    for (int k = 0; k < p.n_orb; ++k)
+        if (s->m[k] < 0 || s->m[k] >= p.n_m || s->m[k] >= p.n_l)
+            return fail(ctx, GGB_E_ARG, "ggb_upload_static: orbital m out of range");
+    for (int i = 0; i < p.n_r; ++i)
+        if (!(s->r[i] > 0.0) || !(s->wr[i] > 0.0))
+            return fail(ctx, GGB_E_ARG, "ggb_upload_static: r and wr must be positive");
+    HIPCHK(ctx, hipSetDevice(p.device));
+    const size_t N = p.n_r, Nl = p.n_l, Nth = p.n_theta;
+    hipStream_t st = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->r, s->r, N * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->wr, s->wr, N * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->v_gs, s->v_gs, N * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->mask, s->mask, N * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->Q, s->Q, Nl * N * p.nk_max * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->lam, s->lam, Nl * p.nk_max * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->nkeep, s->nkeep, Nl * 4, hipMemcpyHostToDevice, st));
+    // B is stored zero padded to [n_m][Nthp][Nlp]
+    for (int mm = 0; mm < p.n_m; ++mm)
+        HIPCHK(ctx, hipMemcpy2DAsync(ctx->B + (size_t)mm * ctx->Nthp * ctx->Nlp, ctx->Nlp * 8,
+                                     s->B + (size_t)mm * Nth * Nl, Nl * 8, Nl * 8, Nth,
+                                     hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->PL, s->PL, Nth * p.n_har * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->chi, s->chi, Nth * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->omega, s->omega, Nth * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->mcol, s->m, p.n_orb * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->occ, s->occ, p.n_orb * 8, hipMemcpyHostToDevice, st));
+    if (s->rho_frozen) {
+        if (!ctx->rho_frozen) HIPCHK(ctx, dalloc(&ctx->rho_frozen, N));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->rho_frozen, s->rho_frozen, N * 8, hipMemcpyHostToDevice, st));
+    } else if (ctx->rho_frozen) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        hipFree(ctx->rho_frozen);
+        ctx->rho_frozen = nullptr;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    ctx->have_static = true;
+    ctx->have_orbitals = false;
+    return GGB_OK;
+}
+
+static AngArgs ang_args(ggb_ctx *ctx, int step) {
+    AngArgs g;
+    g.psi = ctx->psi;
+    g.B = ctx->B;
+    g.mcol = ctx->mcol;
+    g.phase = ctx->phase;
+    g.r = ctx->r;
+    g.chi = ctx->chi;
+    g.E = ctx->E;
+    g.dens = ctx->dens;
+    g.obs_part = ctx->obs_part;
+    g.N = ctx->p.n_r;
+    g.Nl = ctx->p.n_l;
+    g.Nlp = ctx->Nlp;
+    g.Nth = ctx->p.n_theta;
+    g.Nthp = ctx->Nthp;
+    g.C = ctx->C;
+    g.n_orb = ctx->p.n_orb;
+    g.n_field = ctx->n_field;
+    g.step = step;
+    g.dt = ctx->p.dt;
+    return g;
+}
+
+static size_t ang_lds(ggb_ctx *ctx) { return (size_t)(ctx->Nlp + ctx->Nthp) * 16 * sizeof(double); }
+
+// potential rebuild from the |Psi|^2 tiles already in ctx->dens
+static int launch_potential(ggb_ctx *ctx) {
+    const ggb_problem &p = ctx->p;
+    hipStream_t st = ctx->stream;
+    const dim3 gt(ctx->ntiles, p.n_inst);
+    if (p.interacting) {
+        k_density<<<gt, WG, (size_t)RT * p.n_theta * 8, st>>>(
+            ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PL, ctx->vxc,
+            ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
+        k_hartree<<<dim3(p.n_har, p.n_inst), WG, 0, st>>>(ctx->q, ctx->r, ctx->V, p.n_r, p.n_har);
+    }
+    k_dv<<<gt, WG, (size_t)p.n_har * RT * 8, st>>>(ctx->V, ctx->vxc, ctx->PL, ctx->v_gs, ctx->mask,
+                                                   ctx->dv, ctx->phase, p.n_r, p.n_theta, p.n_har,
+                                                   p.dt, p.interacting);
+    HIPCHK(ctx, hipGetLastError());
+    return GGB_OK;
+}
+
+static int launch_first_half(ggb_ctx *ctx, int step) {
+    const dim3 ga(ctx->ntiles, ctx->C);
+    k_angular<0><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+    HIPCHK(ctx, hipGetLastError());
+    return GGB_OK;
+}
+
+static int launch_kinetic(ggb_ctx *ctx) {
+    const ggb_problem &p = ctx->p;
+    const int ncol = (2 * ctx->C + 15) / 16;
+    k_kin_fwd<<<dim3((p.nk_max + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
+        ctx->Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->Y, p.n_r, p.nk_max, ctx->C, p.dt);
+    k_kin_bwd<<<dim3((p.n_r + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
+        ctx->Q, ctx->nkeep, ctx->Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
+    HIPCHK(ctx, hipGetLastError());
+    return GGB_OK;
+}
+
+static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst) {
+    const dim3 ga(ctx->ntiles, ctx->C);
+    k_angular<1><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+    if (obs_dst)
+        k_obs_reduce<<<(ctx->C * 3 + 255) / 256, 256, 0, ctx->stream>>>(ctx->obs_part, obs_dst,
+                                                                         ctx->C, ctx->ntiles);
+    HIPCHK(ctx, hipGetLastError());
+    return launch_potential(ctx);
+}
+
+extern "C" int ggb_upload_orbitals(ggb_ctx *ctx, const double *psi) {
+    if (!ctx) return GGB_E_ARG;
+    if (!psi) return fail(ctx, GGB_E_ARG, "ggb_upload_orbitals: null psi");
+    if (!ctx->have_static) return fail(ctx, GGB_E_STATE, "ggb_upload_orbitals: static data not uploaded");
+    const ggb_problem &p = ctx->p;
+    HIPCHK(ctx, hipSetDevice(p.device));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->psi, psi, (size_t)p.n_l * ctx->C * p.n_r * 16,
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->E) {
+        // a one-sample zero field so that the argument block is valid
+        HIPCHK(ctx, dalloc(&ctx->E, (size_t)p.n_inst));
+        HIPCHK(ctx, hipMemsetAsync(ctx->E, 0, (size_t)p.n_inst * 8, ctx->stream));
+        ctx->n_field = 1;
+    }
+    const dim3 ga(ctx->ntiles, ctx->C);
+    k_angular<2><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, 0));
+    HIPCHK(ctx, hipGetLastError());
+    int rc = launch_potential(ctx);
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_orbitals = true;
+    return GGB_OK;
+}
+
+extern "C" int ggb_set_field(ggb_ctx *ctx, const double *E, int32_t n_steps) {
+    if (!ctx) return GGB_E_ARG;
+    if (!E || n_steps < 1) return fail(ctx, GGB_E_ARG, "ggb_set_field: empty field");
+    const ggb_problem &p = ctx->p;
+    HIPCHK(ctx, hipSetDevice(p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->E) {
+        hipFree(ctx->E);
+        ctx->E = nullptr;
+    }
+    HIPCHK(ctx, dalloc(&ctx->E, (size_t)p.n_inst * n_steps));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->E, E, (size_t)p.n_inst * n_steps * 8, hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->n_field = n_steps;
+    return GGB_OK;
+}
+
+extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip, double *acc,
+                       double *norm) {
+    if (!ctx) return GGB_E_ARG;
+    if (!ctx->have_orbitals) return fail(ctx, GGB_E_STATE, "ggb_run: orbitals not uploaded");
+    if (nsteps < 0 || step0 < 0 || (long)step0 + nsteps > ctx->n_field)
+        return fail(ctx, GGB_E_ARG, "ggb_run: steps outside the uploaded field");
+    if (nsteps == 0) return GGB_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    const size_t need = (size_t)nsteps * ctx->C * 3;
+    if (need > ctx->obs_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->obs) hipFree(ctx->obs);
+        ctx->obs = nullptr;
+        ctx->obs_cap = 0;
+        HIPCHK(ctx, dalloc(&ctx->obs, need));
+        ctx->obs_cap = need;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        const int step = step0 + s;
+        int rc = launch_first_half(ctx, step);
+        if (!rc) rc = launch_kinetic(ctx);
+        if (!rc) rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3);
+        if (rc) return rc;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (dip || acc || norm) {
+        double *h = new double[need];
+        hipError_t e = hipMemcpy(h, ctx->obs, need * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            delete[] h;
+            return fail(ctx, GGB_E_HIP, std::string("ggb_run: copy of observables: ") + hipGetErrorString(e));
+        }
+        for (size_t k = 0; k < (size_t)nsteps * ctx->C; ++k) {
+            if (dip) dip[k] = h[3 * k + 0];
+            if (acc) acc[k] = h[3 * k + 1];
+            if (norm) norm[k] = h[3 * k + 2];
+        }
+        delete[] h;
+    }
+    return GGB_OK;
+}
+
+extern "C" int ggb_download_orbitals(ggb_ctx *ctx, double *psi) {
+    if (!ctx) return GGB_E_ARG;
+    if (!psi) return fail(ctx, GGB_E_ARG, "ggb_download_orbitals: null psi");
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(psi, ctx->psi, (size_t)ctx->p.n_l * ctx->C * ctx->p.n_r * 16,
+                          hipMemcpyDeviceToHost));
+    return GGB_OK;
+}
+
+extern "C" int ggb_download_potential(ggb_ctx *ctx, double *dv) {
+    if (!ctx) return GGB_E_ARG;
+    if (!dv) return fail(ctx, GGB_E_ARG, "ggb_download_potential: null buffer");
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(dv, ctx->dv, (size_t)ctx->p.n_inst * ctx->p.n_r * ctx->p.n_theta * 8,
+                          hipMemcpyDeviceToHost));
+    return GGB_OK;
+}
+
+extern "C" int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step) {
+    if (!ctx) return GGB_E_ARG;
+    if (!ctx->have_orbitals) return fail(ctx, GGB_E_STATE, "ggb_debug_substep: orbitals not uploaded");
+    if (step < 0 || step >= ctx->n_field) return fail(ctx, GGB_E_ARG, "ggb_debug_substep: step outside field");
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    int rc;
+    switch (which) {
+        case 0: rc = launch_first_half(ctx, step); break;
+        case 1: rc = launch_kinetic(ctx); break;
+        case 2: {
+            const dim3 ga(ctx->ntiles, ctx->C);
+            k_angular<1><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+            HIPCHK(ctx, hipGetLastError());
+            rc = GGB_OK;
+            break;
+        }
+        case 3: {
+            const dim3 ga(ctx->ntiles, ctx->C);
+            k_angular<2><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+            HIPCHK(ctx, hipGetLastError());
+            rc = launch_potential(ctx);
+            break;
+        }
+        default: return fail(ctx, GGB_E_ARG, "ggb_debug_substep: unknown sub-step");
+    }
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return GGB_OK;
+}
+
+extern "C" int ggb_selftest_mfma(int32_t device, double *out) {
+    if (!out) return GGB_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return GGB_E_HIP;
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, 256 * sizeof(double)) != hipSuccess) return GGB_E_NOMEM;
+    k_selftest_mfma<<<1, 64>>>(d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, d, 256 * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(d);
+    return e == hipSuccess ? GGB_OK : GGB_E_HIP;
+}

@@ -1,0 +1,124 @@
+"""Propagator: owns one libggbtd context (one HIP stream on one device).
+
+Counterpart of the reference's driver around h_timePropagation
+(main.cu:218-594): allocation, loaders, field, the step loop and the
+observable read-back, with device memory resident across calls.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class Propagator:
+    """Batched propagator for n_inst instances of one AtomSystem.
+
+    Instances share the grid, the ground state and the kinetic tables; each
+    has its own orbitals, field and potentials.
+    """
+
+    def __init__(self, system, n_inst=1, device=0):
+        self.lib = _lib.load()
+        self.sys = system
+        self.n_inst = int(n_inst)
+        self.C = system.n_orb * self.n_inst
+        p = _lib.Problem()
+        p.abi_version = _lib.GGB_ABI_VERSION
+        p.n_r, p.n_l, p.n_theta = system.N, system.n_l, system.n_theta
+        p.n_har, p.n_orb, p.n_inst = system.n_har, system.n_orb, self.n_inst
+        p.n_m, p.nk_max = system.angular.n_m, system.nk_max
+        p.interacting = 1 if system.interacting else 0
+        p.precision, p.device, p.dt = 0, int(device), system.dt
+        self._problem = p
+        ctx = ctypes.c_void_p()
+        _lib.check(self.lib.ggb_create(ctypes.byref(p), ctypes.byref(ctx)), None, "ggb_create")
+        self.ctx = ctx
+        self._arrays = system.static_arrays()
+        a = self._arrays
+        s = _lib.Static()
+        for name in ("r", "wr", "v_gs", "mask", "Q", "lam", "B", "PL", "chi", "omega", "occ"):
+            setattr(s, name, _lib.dptr(a[name]))
+        s.nkeep = _lib.iptr(a["nkeep"])
+        s.m = _lib.iptr(a["m"])
+        s.rho_frozen = None
+        self._check(self.lib.ggb_upload_static(self.ctx, ctypes.byref(s)), "ggb_upload_static")
+        self.n_field = 0
+        self.upload_orbitals()
+
+    def _check(self, rc, what):
+        _lib.check(rc, self.ctx, what)
+
+    # ---- state -----------------------------------------------------------
+    def upload_orbitals(self, psi=None):
+        """psi: [n_l, C, N] complex (default: the ground state in every instance)."""
+        if psi is None:
+            psi = np.concatenate([self.sys.psi0] * self.n_inst, axis=1)
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert psi.shape == (self.sys.n_l, self.C, self.sys.N)
+        buf = psi.view(np.float64)
+        self._check(self.lib.ggb_upload_orbitals(self.ctx, _lib.dptr(buf)), "ggb_upload_orbitals")
+
+    def download_orbitals(self):
+        out = np.empty((self.sys.n_l, self.C, self.sys.N), dtype=np.complex128)
+        self._check(self.lib.ggb_download_orbitals(self.ctx, _lib.dptr(out.view(np.float64))),
+                    "ggb_download_orbitals")
+        return out
+
+    def download_potential(self):
+        out = np.empty((self.n_inst, self.sys.N, self.sys.n_theta))
+        self._check(self.lib.ggb_download_potential(self.ctx, _lib.dptr(out)),
+                    "ggb_download_potential")
+        return out
+
+    # ---- field and propagation ---------------------------------------------
+    def set_field(self, E):
+        """E: [n_inst, n_steps] midpoint field samples (or [n_steps] for all)."""
+        E = np.asarray(E, dtype=np.float64)
+        if E.ndim == 1:
+            E = np.tile(E, (self.n_inst, 1))
+        E = np.ascontiguousarray(E)
+        assert E.shape[0] == self.n_inst
+        self._check(self.lib.ggb_set_field(self.ctx, _lib.dptr(E), E.shape[1]), "ggb_set_field")
+        self.n_field = E.shape[1]
+
+    def run(self, step0, nsteps, observables=True):
+        """Propagate nsteps steps; returns dict(dip, acc, norm) of shape
+        [nsteps, n_inst, n_orb] (None values when observables=False)."""
+        if not observables:
+            self._check(self.lib.ggb_run(self.ctx, step0, nsteps, None, None, None), "ggb_run")
+            return None
+        dip = np.empty((nsteps, self.C))
+        acc = np.empty((nsteps, self.C))
+        nrm = np.empty((nsteps, self.C))
+        self._check(self.lib.ggb_run(self.ctx, step0, nsteps, _lib.dptr(dip), _lib.dptr(acc),
+                                     _lib.dptr(nrm)), "ggb_run")
+        sh = (nsteps, self.n_inst, self.sys.n_orb)
+        return dict(dip=dip.reshape(sh), acc=acc.reshape(sh), norm=nrm.reshape(sh))
+
+    def substep(self, which, step=0):
+        self._check(self.lib.ggb_debug_substep(self.ctx, which, step), "ggb_debug_substep")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.ggb_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def selftest_mfma(device=0):
+    lib = _lib.load()
+    out = np.empty((16, 16))
+    _lib.check(lib.ggb_selftest_mfma(device, _lib.dptr(out)), None, "ggb_selftest_mfma")
+    return out

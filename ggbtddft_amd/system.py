@@ -1,0 +1,102 @@
+"""Static data of one atom on one grid: the precompute stage.
+
+Counterpart of src/scripts/init.py:304-337 (mesh, absorber, ground state,
+radial propagators, angular tables) with the north-star choices:
+* kinetic propagator exp(-i H_l dt), H_l = -T/2 + l(l+1)/2r^2 - Z/r + v_gs,
+  kept in spectral form Q_l diag(e^{-i lam dt}) Q_l^T on the eigenpairs with
+  lam <= E_cut = ecut_factor * pi / dt (components above turn by more than pi
+  per step, are not resolved by the splitting, and with the theta-dependent
+  LDA they resonate at the innermost nodes; dropping them keeps the scheme
+  stable and makes the streamed tables smaller);
+* absorber: Manolopoulos CAP of internal.py:460-475 (getABCR, ABC_TYPE 1) on
+  [r_max/1.5, r_max] (R_MAX = 1.5 R_MASK, init.py:284), applied once per step
+  as the mask exp(-W dt/2) in each potential half step;
+* theta grid of 2 Lmax + 1 Gauss-Legendre nodes and Hartree multipoles up to
+  L_H = 2 Lmax by default (exact multipoles of a density built from l <= Lmax).
+"""
+import math
+
+import numpy as np
+
+from .angular import AngularTables
+from .atoms import Atom
+from .grid import RadialGrid
+from .groundstate import GroundState
+
+
+def manolopoulos_cap(r, r_mask, r_max):
+    delta, c = 0.20, 2.62206
+    a = 1.0 - 16.0 / c ** 3
+    b = (1.0 - 17.0 / c ** 3) / c ** 2
+    kmin = c / (2.0 * delta * (r_max - r_mask))
+    emin = 0.5 * kmin * kmin
+    W = np.zeros_like(r)
+    i0 = int(np.argmin(np.abs(r - r_mask)))
+    d = r[i0:] - r_mask
+    x = 2.0 * delta * kmin * d
+    W[i0:] = emin * d * (a * x - b * x ** 3 + 4.0 / (c - x) ** 2 - 4.0 / (c + x) ** 2)
+    return W
+
+
+def _round16(n):
+    return ((int(n) + 15) // 16) * 16
+
+
+class AtomSystem:
+    def __init__(self, atom, N, r_max, lmax, dt, zeta=1.0, n_theta=None, l_h=None, seed=0,
+                 cap=True, interacting=None, ecut_factor=1.0, ground_state=None):
+        self.atom = atom if isinstance(atom, Atom) else Atom(atom)
+        self.N, self.r_max, self.lmax, self.dt = int(N), float(r_max), int(lmax), float(dt)
+        self.n_l = self.lmax + 1
+        self.l_h = 2 * self.lmax if l_h is None else int(l_h)
+        self.n_har = self.l_h + 1
+        self.n_theta = 2 * self.lmax + 1 if n_theta is None else int(n_theta)
+        self.grid = RadialGrid(self.N, self.r_max, zeta)
+        orbs = self.atom.orbitals
+        self.n_orb = len(orbs)
+        self.m = np.array([o[2] for o in orbs], dtype=np.int32)
+        self.occ = np.array([o[3] for o in orbs], dtype=np.float64)
+        self.angular = AngularTables(self.lmax, self.n_theta, self.l_h, self.m)
+        self.gs = ground_state if ground_state is not None else GroundState(
+            self.grid, self.atom, interacting=interacting, seed=seed)
+        self.interacting = self.gs.interacting
+        self.v_gs = self.gs.v_gs
+        self.ecut = ecut_factor * math.pi / self.dt
+        eig = []
+        for l in range(self.n_l):
+            lam, Q = np.linalg.eigh(self.grid.hamiltonian(l, self.atom.Z, self.v_gs))
+            nk = int(np.searchsorted(lam, self.ecut, side="right"))
+            eig.append((lam[:nk], Q[:, :nk]))
+        self.nkeep = np.array([len(e[0]) for e in eig], dtype=np.int32)
+        self.nk_max = _round16(max(1, self.nkeep.max()))
+        self.Q = np.zeros((self.n_l, self.N, self.nk_max))
+        self.lam = np.zeros((self.n_l, self.nk_max))
+        for l, (lam, Q) in enumerate(eig):
+            self.Q[l, :, :lam.size] = Q
+            self.lam[l, :lam.size] = lam
+        self.W = manolopoulos_cap(self.grid.r, self.r_max / 1.5, self.r_max) if cap else np.zeros(self.N)
+        self.mask = np.exp(-0.5 * self.dt * self.W)
+        self.psi0 = np.zeros((self.n_l, self.n_orb, self.N), dtype=np.complex128)
+        for k, (n, l, m, occ, s) in enumerate(orbs):
+            self.psi0[l, k] = self.gs.u[s]
+
+    def static_arrays(self):
+        """Contiguous arrays in the layouts of ggb_static (include/ggbtd.h)."""
+        a = self.angular
+        return dict(r=np.ascontiguousarray(self.grid.r), wr=np.ascontiguousarray(self.grid.wr),
+                    v_gs=np.ascontiguousarray(self.v_gs), mask=np.ascontiguousarray(self.mask),
+                    Q=np.ascontiguousarray(self.Q), lam=np.ascontiguousarray(self.lam),
+                    nkeep=np.ascontiguousarray(self.nkeep), B=np.ascontiguousarray(a.B),
+                    PL=np.ascontiguousarray(a.PL), chi=np.ascontiguousarray(a.x),
+                    omega=np.ascontiguousarray(a.omega), m=np.ascontiguousarray(self.m),
+                    occ=np.ascontiguousarray(self.occ))
+
+    def total_acceleration(self, acc, norm, E):
+        """Ehrenfest dipole acceleration of all electrons,
+        a(t) = -Z sum_k occ_k <cos(theta)/r^2>_k - E(t) sum_k occ_k <psi_k|psi_k>
+        (Hartree and LDA forces cancel in the sum over electrons)."""
+        return -self.atom.Z * (acc @ self.occ) - E * (norm @ self.occ)
+
+    def table_bytes(self):
+        """Bytes of the eigenvector tables streamed per step (FP64)."""
+        return int(self.nkeep.sum()) * self.N * 8

@@ -1,0 +1,136 @@
+/*
+ * ggbtd.h -- C ABI of the MI355X real-time TDDFT propagator (libggbtd.so).
+ *
+ * The reference (graham-g-brown/ggbTDDFT) has no library interface: its hot
+ * path is the C++ function
+ *     void h_timePropagation(ksoSetTDStruct, hartreeStruct, propagatorsStruct,
+ *                            meshStruct, observableStruct, tempStruct,
+ *                            handleStruct, int scandx)
+ * (src/subFunctions/timePropagation/h_timePropagation.cuh:32-41), fed by
+ * loaders that read workingData/ files and by compile-time macros in
+ * params.cuh.  Each entry point below names the reference piece it replaces.
+ *
+ * Conventions: every function returns 0 on success and a negative GGB_E*
+ * code on failure (never exit()); ggb_last_error() gives the message.  Host
+ * buffers are caller-owned and copied; device memory is owned by the context.
+ * One context = one HIP stream on one device.  A context may be used from
+ * one thread at a time (thread-compatible, not thread-safe).
+ *
+ * Layouts (all little-endian, row-major, double = IEEE binary64):
+ *   state    psi[l][c][i] complex (re, im interleaved), c = inst * n_orb + orb,
+ *            l = 0..n_l-1, i = 0..n_r-1, zero for l < m_orb
+ *            (the reference's ksoSetTD.d_x[l][i + k N_r], main.cu:363-391).
+ *   field    E[inst][step] at the step midpoint t_n + dt/2.
+ *   observables per call: dip/acc/norm[step - step0][c].
+ */
+#ifndef GGBTD_H
+#define GGBTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GGB_ABI_VERSION 1
+
+#define GGB_OK 0
+#define GGB_E_ARG (-1)      /* invalid argument / size */
+#define GGB_E_HIP (-2)      /* HIP runtime error */
+#define GGB_E_STATE (-3)    /* call out of order (e.g. run before upload) */
+#define GGB_E_NOMEM (-4)    /* device allocation failed */
+
+typedef struct ggb_ctx ggb_ctx;
+
+/* Sizes and switches of one context.  Replaces the params.cuh macros N_r,
+ * N_l, N_kso, N_har, dt, USE_HARTREE (init.py:493-604). */
+typedef struct ggb_problem {
+    int32_t abi_version;  /* must be GGB_ABI_VERSION */
+    int32_t n_r;          /* radial Gauss-Lobatto nodes N (>= 4) */
+    int32_t n_l;          /* angular momenta, Lmax + 1 (<= 128) */
+    int32_t n_theta;      /* Gauss-Legendre theta nodes (>= n_l, <= 256) */
+    int32_t n_har;        /* Hartree multipoles L = 0..n_har-1 (<= 256) */
+    int32_t n_orb;        /* orbitals per instance */
+    int32_t n_inst;       /* instances sharing the static data */
+    int32_t n_m;          /* distinct |m| values; orbital m in [0, n_m) */
+    int32_t nk_max;       /* row length of the eigenvector table, multiple of 16 */
+    int32_t interacting;  /* 1: Hartree + LDA update each step; 0: dV = 0 */
+    int32_t precision;    /* 0: FP64 (only value accepted by this version) */
+    int32_t device;       /* HIP device ordinal */
+    double dt;            /* time step (a.u.) */
+} ggb_problem;
+
+/* Static per-grid data.  Replaces the loaders h_initialiseMesh
+ * (h_initialiseMesh.cuh:1-81) and h_initialisePropagators
+ * (h_initialisePropagators.cuh:13-118). */
+typedef struct ggb_static {
+    const double *r;           /* [n_r] radial nodes */
+    const double *wr;          /* [n_r] Lobatto weight * dr/dx */
+    const double *v_gs;        /* [n_r] ground-state KS potential in the kinetic propagator */
+    const double *mask;        /* [n_r] absorber exp(-W_cap dt/2) per half step */
+    const double *Q;           /* [n_l][n_r][nk_max] eigenvectors of H_l as columns, zero padded */
+    const double *lam;         /* [n_l][nk_max] eigenvalues (padding ignored) */
+    const int32_t *nkeep;      /* [n_l] retained eigenpairs per l (<= nk_max) */
+    const double *B;           /* [n_m][n_theta][n_l] sqrt(omega_j) Theta_lm(chi_j) */
+    const double *PL;          /* [n_theta][n_har] Legendre P_L(chi_j) */
+    const double *chi;         /* [n_theta] cos(theta_j) */
+    const double *omega;       /* [n_theta] Gauss-Legendre weights */
+    const int32_t *m;          /* [n_orb] |m| of each orbital */
+    const double *occ;         /* [n_orb] occupations */
+    const double *rho_frozen;  /* [n_r] frozen-core u-unit density, or NULL */
+} ggb_static;
+
+int ggb_abi_version(void);
+
+/* Create a context and allocate its device memory. */
+int ggb_create(const ggb_problem *p, ggb_ctx **out);
+
+/* Upload mesh, propagator and angular tables. */
+int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s);
+
+/* Upload orbitals psi[l][c][i] (complex interleaved, c over all instances)
+ * and build the potential of that state.  Replaces h_initialiseTDKSOSet
+ * (h_initialiseTDKSOSet.cuh:19-107) and the t=0 potentials of
+ * h_timePropagation.cuh:47-57. */
+int ggb_upload_orbitals(ggb_ctx *ctx, const double *psi);
+
+/* Upload the field E[inst][step] for n_steps steps.  Replaces
+ * h_initialiseField (h_initialiseField.cuh:1-40). */
+int ggb_set_field(ggb_ctx *ctx, const double *E, int32_t n_steps);
+
+/* Propagate steps step0 .. step0+nsteps-1 and copy the per-step observables
+ * <r cos theta>, <cos theta / r^2> and <psi|psi> of every orbital column into
+ * dip/acc/norm[nsteps][n_inst*n_orb] (any may be NULL).  Replaces the step
+ * loop of h_timePropagation (h_timePropagation.cuh:59-112) with the
+ * observables of dipoleMoment.cuh / dipoleAcceleration.cuh. */
+int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip, double *acc,
+            double *norm);
+
+/* Copy the current orbitals out (same layout as ggb_upload_orbitals). */
+int ggb_download_orbitals(ggb_ctx *ctx, double *psi);
+
+/* Copy the current potential dV[inst][i][j] = v_H + v_xc - v_gs on the
+ * (r, theta) grid (the one the next step applies). */
+int ggb_download_potential(ggb_ctx *ctx, double *dv);
+
+/* Run one sub-step in isolation (for parity tests):
+ * 0 = half potential then laser even/odd, 1 = kinetic propagator,
+ * 2 = laser odd/even, half potential, density, 3 = potential rebuild from the
+ * current state.  `step` selects the field sample. */
+int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step);
+
+/* 16x16x4 FP64 matrix-core product of fixed inputs A[i][k] = i + 0.25 k,
+ * B[k][j] = 0.5 k - j, written to out[16][16] (lane-layout self test). */
+int ggb_selftest_mfma(int32_t device, double *out);
+
+/* Message of the last failure on ctx (or of the last failed ggb_create when
+ * ctx is NULL). */
+const char *ggb_last_error(const ggb_ctx *ctx);
+
+void ggb_destroy(ggb_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GGBTD_H */
