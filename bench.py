@@ -1,0 +1,153 @@
+"""Benchmark: orbital-timesteps/s of the FP64 propagator on one MI355X.
+
+Workload (the metric's configuration): argon all-electron, 7 orbitals grouped
+by |m| (m=0: 1s 2s 2p0 3s 3p0, |m|=1: 2p1 3p1 at double weight), N = 512
+Gauss-Lobatto radial nodes to r_max = 200, Lmax = 64, Hartree multipoles to
+L = 128 on 129 theta nodes, 800 nm sin^2 pulse of 8 cycles at 2e14 W/cm^2,
+dt = 0.02 (44128 steps per full run), FP64 throughout.  A "step" is one time
+step of the whole split-operator update (potential half steps, laser, kinetic
+propagator, potential rebuild, observables) for the instance.
+
+value = orbital-timesteps/s = 7 * K / (wall time of K steps), state and
+tables resident in HBM; the K steps are bracketed by stream synchronisation.
+The dominant kernel (the kinetic propagator pass) is timed with HIP events on
+its stream inside the timed region for the roofline entry.  The CPU baseline
+is the FP64 oracle on the host cores for a bounded window of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+ATOM, N, R_MAX, LMAX, DT = "Ar", 512, 200.0, 64, 0.02
+INTENSITY, WAVELENGTH, CYCLES = 2e14, 800.0, 8
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(system_args, E, nsteps=40, warm=3):
+    """FP64 oracle on the host for a bounded window (setup excluded)."""
+    import contextlib
+    from oracle import propagate as OP
+    cores = min(16, os.cpu_count() or 1)
+    try:
+        from threadpoolctl import threadpool_limits
+        limit = threadpool_limits(limits=cores)
+    except Exception:  # pragma: no cover
+        limit = contextlib.nullcontext()
+    with limit:
+        t0 = time.perf_counter()
+        S = OP.System(*system_args)
+        setup = time.perf_counter() - t0
+        psi = S.psi0.copy()
+        res = S.run(E[:warm], psi=psi)
+        t0 = time.perf_counter()
+        S.run(E[warm:warm + nsteps], psi=res["psi"])
+        el = time.perf_counter() - t0
+    log("cpu baseline: %d steps in %.1f s" % (nsteps, el))
+    return dict(value=S.norb * nsteps / el, unit="orbital-timesteps/s", cores=cores,
+                kind="port",
+                sample="oracle (numpy FP64), Ar N=512 Lmax=64, %d steps after %d warm-up "
+                       "steps, %.1f s (setup %.1f s excluded)" % (nsteps, warm, el, setup))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs single-device instances "
+              "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from ggbtddft_amd.engine import Propagator
+    from ggbtddft_amd.pulse import Pulse
+    from ggbtddft_amd.system import AtomSystem
+
+    t0 = time.perf_counter()
+    sysm = AtomSystem(ATOM, N, R_MAX, LMAX, DT)
+    log("precompute %.1f s (ground state %d iterations, sum nkeep %d)"
+        % (time.perf_counter() - t0, sysm.gs.iterations, int(sysm.nkeep.sum())))
+    pulse = Pulse(INTENSITY, WAVELENGTH, CYCLES)
+    nfull = pulse.n_steps(DT)
+    need = args.warmup + args.steps
+    if need > nfull:
+        raise SystemExit("warmup + steps (%d) exceeds the pulse (%d steps)" % (need, nfull))
+    E = pulse.midpoints(DT)
+
+    prop = Propagator(sysm)
+    prop.set_field(E)
+    prop.run(0, args.warmup)
+    log("warm-up done (%d steps)" % args.warmup)
+    prop.set_timing(1)
+    t0 = time.perf_counter()
+    obs = prop.run(args.warmup, args.steps)
+    el = time.perf_counter() - t0
+    log("timed %d steps in %.2f s" % (args.steps, el))
+    tim = prop.timing()
+    psi = prop.download_orbitals()
+    prop.close()
+    assert np.all(np.isfinite(obs["acc"])) and np.all(np.isfinite(psi))
+
+    norb = sysm.n_orb
+    value = norb * args.steps / el
+    ms_step = 1e3 * el / args.steps
+    # kinetic pass with the Q^T table: reads sum_l nkeep_l N doubles of Q,
+    # the state (n_l C N complex) and writes Y (sum_l nkeep_l C complex)
+    C = norb
+    nk = int(sysm.nkeep.sum())
+    bytes_qt = nk * N * 8 + sysm.n_l * C * N * 16 + nk * C * 16
+    ms_qt, n_qt = tim["kinetic_qt"]
+    ms_q, n_q = tim["kinetic_q"]
+    avg_qt = ms_qt / max(n_qt, 1)
+    avg_q = ms_q / max(n_q, 1)
+    achieved = bytes_qt / (avg_qt * 1e-3) / 1e9
+    line = {
+        "metric": "orbital-timesteps/s, Ar N=512 Lmax=64: batched FP32 vs FP64; p99 time-to-spectrum",
+        "value": value,
+        "unit": "orbital-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded imaginary-time ground state, analytic sin^2 pulse)",
+        "config": {"workload": "argon all-electron FP64 plain, one instance",
+                   "atom": ATOM, "n_r": N, "r_max": R_MAX, "lmax": LMAX, "l_hartree": 2 * LMAX,
+                   "n_theta": sysm.n_theta, "orbitals": norb, "dt": DT,
+                   "pulse": "800 nm sin^2 8 cycles 2e14 W/cm^2", "full_run_steps": nfull,
+                   "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": {"kernel": "k_kin_fwd (kinetic propagator, Q^T pass)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_qt, "avg_launch_ms": avg_qt,
+                     "avg_launch_ms_q_pass": avg_q},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline((ATOM, N, R_MAX, LMAX, DT), E,
+                                                nsteps=args.cpu_steps)
+        except Exception as exc:  # the GPU number stands on its own
+            line["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

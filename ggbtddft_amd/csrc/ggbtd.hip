@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "ggbtd.h"
 
@@ -55,9 +56,12 @@ static __device__ __forceinline__ double c_lm(int l, int m) {
 }
 
 // ---------------------------------------------------------------------------
-// Kinetic propagator, pass 1: Y[l][c][a] = e^{-i lam_a dt} sum_i Q[l][i][a] psi[l][c][i]
-// M = eigen index a, N = real column (c, re/im), K = radial node i.
+// Kinetic propagator, pass 1: Y[l][c][p][a] = (e^{-i lam_a dt} sum_i Q[l][i][a] psi[l][c][i])_p
+// M = eigen index a, N = real column (c, p = re/im), K = radial node i.
+// Y is planar in (re, im) so that pass 2 reads contiguous runs of a.
+// Each lane keeps KU loads of A and B in flight before the MFMAs.
 // ---------------------------------------------------------------------------
+#define KU 16
 __global__ __launch_bounds__(WG) void k_kin_fwd(const double *__restrict__ Q,
                                                 const double *__restrict__ lam,
                                                 const int32_t *__restrict__ nkeep,
@@ -73,18 +77,23 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const double *__restrict__ Q,
     const int col = blockIdx.y * 16 + lc;
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
-    const double *Ql = Q + (size_t)l * N * nk_max;
-    const double *Xl = psi + (size_t)l * C * N * 2;
-    const int am = a0 + lc;
+    const double *Ql = Q + (size_t)l * N * nk_max + a0 + lc;
+    const double *Xl = psi + ((size_t)l * C + (colok ? c : 0)) * N * 2 + p;
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-    for (int k0 = 0; k0 < N; k0 += 4) {
-        const int i = k0 + kr;
-        const double av = (i < N) ? Ql[(size_t)i * nk_max + am] : 0.0;
-        const double bv = (colok && i < N) ? Xl[((size_t)c * N + i) * 2 + p] : 0.0;
-        acc = mfma64(av, bv, acc);
+    for (int k0 = 0; k0 < N; k0 += 4 * KU) {
+        double av[KU], bv[KU];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int i = k0 + 4 * u + kr;
+            const bool ok = i < N;
+            av[u] = ok ? Ql[(size_t)i * nk_max] : 0.0;
+            bv[u] = (ok && colok) ? Xl[(size_t)i * 2] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < KU; ++u) acc = mfma64(av[u], bv[u], acc);
     }
     const double *laml = lam + (size_t)l * nk_max;
+    double o[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int a = a0 + 4 * kr + r;
@@ -93,12 +102,20 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const double *__restrict__ Q,
         const double re = p ? vo : v, im = p ? v : vo;
         double sn, cs;
         sincos(laml[a < nk ? a : 0] * dt, &sn, &cs);
-        const double o = p ? (cs * im - sn * re) : (cs * re + sn * im);
-        if (colok && a < nk) Y[(((size_t)l * C + c) * nk_max + a) * 2 + p] = o;
+        o[r] = p ? (cs * im - sn * re) : (cs * re + sn * im);
+    }
+    if (colok) {
+        double *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max + a0 + 4 * kr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (a0 + 4 * kr + r < nk) Yo[r] = o[r];
     }
 }
 
-// Kinetic propagator, pass 2: psi[l][c][i] = sum_a Q[l][i][a] Y[l][c][a]
+// Kinetic propagator, pass 2: psi[l][c][i] = sum_a Q[l][i][a] Y[l][c][a].
+// K order is permuted (lane kr of MFMA j in chunk u takes a = k0 + 16u + 4kr + j)
+// so that each lane reads 4 consecutive a of Q and of planar Y.
+#define KU2 4
 __global__ __launch_bounds__(WG) void k_kin_bwd(const double *__restrict__ Q,
                                                 const int32_t *__restrict__ nkeep,
                                                 const double *__restrict__ Y,
@@ -113,17 +130,25 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const double *__restrict__ Q,
     const int col = blockIdx.y * 16 + lc;
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
-    const double *Ql = Q + (size_t)l * N * nk_max;
-    const double *Yl = Y + (size_t)l * C * nk_max * 2;
     const int im_ = i0 + lc;
     const bool rowok = im_ < N;
+    const double *Ql = Q + (size_t)l * N * nk_max + (size_t)(rowok ? im_ : 0) * nk_max;
+    const double *Yl = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max;
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-    for (int k0 = 0; k0 < nk; k0 += 4) {
-        const int a = k0 + kr;
-        const double av = (rowok && a < nk) ? Ql[(size_t)im_ * nk_max + a] : 0.0;
-        const double bv = (colok && a < nk) ? Yl[((size_t)c * nk_max + a) * 2 + p] : 0.0;
-        acc = mfma64(av, bv, acc);
+    for (int k0 = 0; k0 < nk; k0 += 16 * KU2) {
+        double av[4 * KU2], bv[4 * KU2];
+#pragma unroll
+        for (int u = 0; u < KU2; ++u) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int a = k0 + 16 * u + 4 * kr + j;
+                const bool ok = a < nk;
+                av[4 * u + j] = (ok && rowok) ? Ql[a] : 0.0;
+                bv[4 * u + j] = (ok && colok) ? Yl[a] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4 * KU2; ++u) acc = mfma64(av[u], bv[u], acc);
     }
     double *Xl = psi + (size_t)l * C * N * 2;
 #pragma unroll
@@ -177,6 +202,8 @@ static __device__ void ang_laser(double *X, int parity, int m, int Nl, const dou
 }
 
 // theta-space product P = B X; accumulators for up to 4 M-tiles per wave.
+// K order permuted (lane kr of MFMA j takes l = k0 + 4kr + j) so that each
+// lane reads 4 consecutive l of its row of B; 8 loads in flight per lane.
 static __device__ void ang_fwd(const double *Bm, const double *X, int Nlp, int Nthp, d4_t *acc) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kr = lane >> 4, lc = lane & 15;
@@ -185,16 +212,26 @@ static __device__ void ang_fwd(const double *Bm, const double *X, int Nlp, int N
         acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
         const int tile = wave + 4 * t;
         if (tile >= mt) continue;
-        const double *Arow = Bm + (size_t)(tile * 16 + lc) * Nlp;
+        const double *Arow = Bm + (size_t)(tile * 16 + lc) * Nlp + 4 * kr;
         d4_t a = acc[t];
-        for (int k0 = 0; k0 < Nlp; k0 += 4) {
-            a = mfma64(Arow[k0 + kr], X[(k0 + kr) * 16 + lc], a);
+        for (int k0 = 0; k0 < Nlp; k0 += 32) {
+            double av[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int kb = k0 + 16 * (u >> 2) + (u & 3);
+                av[u] = (kb < Nlp) ? Arow[kb] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int kb = k0 + 16 * (u >> 2) + (u & 3);
+                if (kb < Nlp) a = mfma64(av[u], X[(kb + 4 * kr) * 16 + lc], a);
+            }
         }
         acc[t] = a;
     }
 }
 
-// l-space product X = B^T P; up to 2 M-tiles per wave.
+// l-space product X = B^T P; up to 2 M-tiles per wave, 8 loads in flight.
 static __device__ void ang_bwd(const double *Bm, const double *P, int Nlp, int Nthp, d4_t *acc) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kr = lane >> 4, lc = lane & 15;
@@ -203,9 +240,20 @@ static __device__ void ang_bwd(const double *Bm, const double *P, int Nlp, int N
         acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
         const int tile = wave + 4 * t;
         if (tile >= mt) continue;
+        const double *Acol = Bm + (size_t)kr * Nlp + tile * 16 + lc;
         d4_t a = acc[t];
-        for (int k0 = 0; k0 < Nthp; k0 += 4) {
-            a = mfma64(Bm[(size_t)(k0 + kr) * Nlp + tile * 16 + lc], P[(k0 + kr) * 16 + lc], a);
+        for (int k0 = 0; k0 < Nthp; k0 += 32) {
+            double av[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + 4 * u;
+                av[u] = (k < Nthp) ? Acol[(size_t)k * Nlp] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + 4 * u;
+                if (k < Nthp) a = mfma64(av[u], P[(k + kr) * 16 + lc], a);
+            }
         }
         acc[t] = a;
     }
@@ -571,6 +619,11 @@ struct ggb_ctx {
     double *dens, *vxc, *q, *V, *dv, *obs_part, *obs, *E;
     double2 *phase;
     size_t obs_cap;
+    // optional HIP-event timing of the step phases (ggb_set_timing)
+    int timing;                     // 0 off, 1 kinetic only, 2 every phase
+    std::vector<hipEvent_t> events; // GGB_NPHASE + 1 per step of a run call
+    double phase_ms[GGB_NPHASE];
+    long phase_n[GGB_NPHASE];
 };
 
 static std::string g_create_err;
@@ -612,6 +665,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase};
     for (void *b : bufs)
         if (b) hipFree(b);
+    for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -650,6 +704,11 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->dens = ctx->vxc = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
     ctx->phase = nullptr;
     ctx->stream = nullptr;
+    ctx->timing = 0;
+    for (int k = 0; k < GGB_NPHASE; ++k) {
+        ctx->phase_ms[k] = 0.0;
+        ctx->phase_n[k] = 0;
+    }
 
     auto bail = [&](const std::string &m, int code) {
         g_create_err = m;
@@ -796,15 +855,22 @@ static int launch_first_half(ggb_ctx *ctx, int step) {
     return GGB_OK;
 }
 
-static int launch_kinetic(ggb_ctx *ctx) {
+static int launch_kinetic_pass(ggb_ctx *ctx, int pass) {
     const ggb_problem &p = ctx->p;
     const int ncol = (2 * ctx->C + 15) / 16;
-    k_kin_fwd<<<dim3((p.nk_max + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
-        ctx->Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->Y, p.n_r, p.nk_max, ctx->C, p.dt);
-    k_kin_bwd<<<dim3((p.n_r + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
-        ctx->Q, ctx->nkeep, ctx->Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
+    if (pass == 0)
+        k_kin_fwd<<<dim3((p.nk_max + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
+            ctx->Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->Y, p.n_r, p.nk_max, ctx->C, p.dt);
+    else
+        k_kin_bwd<<<dim3((p.n_r + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
+            ctx->Q, ctx->nkeep, ctx->Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
+}
+
+static int launch_kinetic(ggb_ctx *ctx) {
+    int rc = launch_kinetic_pass(ctx, 0);
+    return rc ? rc : launch_kinetic_pass(ctx, 1);
 }
 
 static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst) {
@@ -876,14 +942,46 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         HIPCHK(ctx, dalloc(&ctx->obs, need));
         ctx->obs_cap = need;
     }
+    const int ne = GGB_NPHASE + 1;
+    if (ctx->timing) {
+        const size_t want = (size_t)nsteps * ne;
+        while (ctx->events.size() < want) {
+            hipEvent_t e;
+            HIPCHK(ctx, hipEventCreate(&e));
+            ctx->events.push_back(e);
+        }
+    }
+    auto mark = [&](int s, int k) -> hipError_t {
+        if (!ctx->timing) return hipSuccess;
+        if (ctx->timing == 1 && k != 1 && k != 2 && k != 3) return hipSuccess;
+        return hipEventRecord(ctx->events[(size_t)s * ne + k], ctx->stream);
+    };
     for (int s = 0; s < nsteps; ++s) {
         const int step = step0 + s;
+        HIPCHK(ctx, mark(s, 0));
         int rc = launch_first_half(ctx, step);
-        if (!rc) rc = launch_kinetic(ctx);
+        HIPCHK(ctx, mark(s, 1));
+        if (!rc) rc = launch_kinetic_pass(ctx, 0);
+        HIPCHK(ctx, mark(s, 2));
+        if (!rc) rc = launch_kinetic_pass(ctx, 1);
+        HIPCHK(ctx, mark(s, 3));
         if (!rc) rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3);
+        HIPCHK(ctx, mark(s, 4));
         if (rc) return rc;
     }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->timing) {
+        for (int s = 0; s < nsteps; ++s) {
+            for (int k = 0; k < GGB_NPHASE; ++k) {
+                if (ctx->timing == 1 && k != 1 && k != 2) continue;
+                float ms = 0.f;
+                HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->events[(size_t)s * ne + k],
+                                                ctx->events[(size_t)s * ne + k + 1]));
+                ctx->phase_ms[k] += ms;
+                ctx->phase_n[k] += 1;
+            }
+        }
+    }
     if (dip || acc || norm) {
         double *h = new double[need];
         hipError_t e = hipMemcpy(h, ctx->obs, need * 8, hipMemcpyDeviceToHost);
@@ -961,4 +1059,24 @@ extern "C" int ggb_selftest_mfma(int32_t device, double *out) {
     if (e == hipSuccess) e = hipMemcpy(out, d, 256 * sizeof(double), hipMemcpyDeviceToHost);
     hipFree(d);
     return e == hipSuccess ? GGB_OK : GGB_E_HIP;
+}
+
+extern "C" int ggb_set_timing(ggb_ctx *ctx, int32_t mode) {
+    if (!ctx) return GGB_E_ARG;
+    if (mode < 0 || mode > 2) return fail(ctx, GGB_E_ARG, "ggb_set_timing: mode must be 0, 1 or 2");
+    ctx->timing = mode;
+    for (int k = 0; k < GGB_NPHASE; ++k) {
+        ctx->phase_ms[k] = 0.0;
+        ctx->phase_n[k] = 0;
+    }
+    return GGB_OK;
+}
+
+extern "C" int ggb_get_timing(const ggb_ctx *ctx, double *ms_total, int64_t *count) {
+    if (!ctx || !ms_total || !count) return GGB_E_ARG;
+    for (int k = 0; k < GGB_NPHASE; ++k) {
+        ms_total[k] = ctx->phase_ms[k];
+        count[k] = ctx->phase_n[k];
+    }
+    return GGB_OK;
 }

@@ -96,6 +96,19 @@ class Propagator:
         sh = (nsteps, self.n_inst, self.sys.n_orb)
         return dict(dip=dip.reshape(sh), acc=acc.reshape(sh), norm=nrm.reshape(sh))
 
+    def set_timing(self, mode):
+        """0 off, 1 kinetic passes, 2 every phase (HIP events on the context stream)."""
+        self._check(self.lib.ggb_set_timing(self.ctx, int(mode)), "ggb_set_timing")
+
+    def timing(self):
+        """{phase: (total ms, launches)} accumulated since set_timing."""
+        ms = np.zeros(_lib.NPHASE)
+        n = np.zeros(_lib.NPHASE, dtype=np.int64)
+        self._check(self.lib.ggb_get_timing(self.ctx, _lib.dptr(ms),
+                                            n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+                    "ggb_get_timing")
+        return {name: (float(ms[k]), int(n[k])) for k, name in enumerate(_lib.PHASES)}
+
     def substep(self, which, step=0):
         self._check(self.lib.ggb_debug_substep(self.ctx, which, step), "ggb_debug_substep")
 

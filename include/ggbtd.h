@@ -119,6 +119,20 @@ int ggb_download_potential(ggb_ctx *ctx, double *dv);
  * current state.  `step` selects the field sample. */
 int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step);
 
+/* Step phases timed by ggb_set_timing: 0 = half potential + laser,
+ * 1 = kinetic pass Q^T, 2 = kinetic pass Q, 3 = laser + half potential +
+ * observables, 4 = potential rebuild. */
+#define GGB_NPHASE 5
+
+/* Record HIP events on the context stream around the step phases of later
+ * ggb_run calls (mode 1: kinetic passes only, 2: every phase, 0: off) and
+ * reset the accumulated times.  No counterpart in the reference, which has
+ * no timing code. */
+int ggb_set_timing(ggb_ctx *ctx, int32_t mode);
+
+/* Accumulated milliseconds and launch counts per phase [GGB_NPHASE]. */
+int ggb_get_timing(const ggb_ctx *ctx, double *ms_total, int64_t *count);
+
 /* 16x16x4 FP64 matrix-core product of fixed inputs A[i][k] = i + 0.25 k,
  * B[k][j] = 0.5 k - j, written to out[16][16] (lane-layout self test). */
 int ggb_selftest_mfma(int32_t device, double *out);

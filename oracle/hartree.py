@@ -37,21 +37,19 @@ def direct(q, r):
 
 
 def scan(q, r):
+    """Sequential in r, vectorised over L."""
     q = np.asarray(q, dtype=np.float64)
     nL, N = q.shape
+    Ls = np.arange(nL, dtype=np.float64)
     V = np.zeros_like(q)
-    for L in range(nL):
-        A = np.zeros(N)
-        acc = 0.0
-        for a in range(N):
-            if a > 0:
-                acc *= (r[a - 1] / r[a]) ** (L + 1)
-            acc += q[L, a] / r[a]
-            A[a] = acc
-        B = np.zeros(N)
-        acc = 0.0
-        for a in range(N - 2, -1, -1):
-            acc = (r[a] / r[a + 1]) ** L * (acc + q[L, a + 1] / r[a + 1])
-            B[a] = acc
-        V[L] = A + B
+    acc = np.zeros(nL)
+    for a in range(N):
+        if a > 0:
+            acc = acc * (r[a - 1] / r[a]) ** (Ls + 1.0)
+        acc = acc + q[:, a] / r[a]
+        V[:, a] = acc
+    acc = np.zeros(nL)
+    for a in range(N - 2, -1, -1):
+        acc = (r[a] / r[a + 1]) ** Ls * (acc + q[:, a + 1] / r[a + 1])
+        V[:, a] += acc
     return V
