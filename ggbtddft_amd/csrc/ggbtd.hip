@@ -44,6 +44,7 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 #define WG 256
 #define RT 8          // radial nodes per angular tile (16 real columns)
 #define RHO_MIN 1e-40 // v_xc = 0 at or below this density
+#define KIN_NARROW_MAX_C 8  // up to 8 complex columns: one 16-wide MFMA column tile
 
 static __device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -155,6 +156,200 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const double *__restrict__ Q,
     for (int r = 0; r < 4; ++r) {
         const int i = i0 + 4 * kr + r;
         if (colok && i < N) Xl[((size_t)c * N + i) * 2 + p] = acc[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled kinetic passes for many columns (instances sharing the tables).
+// Work-group tile 64 (M) x 64 (real columns), K chunks of 16 staged in LDS
+// with a register prefetch of the next chunk; wave w owns the 32 x 32
+// sub-tile (w >> 1, w & 1) as 2 x 2 MFMA tiles.  Same element-wise
+// summation order over K as the narrow kernels above (pass 1).
+// ---------------------------------------------------------------------------
+#define TB 64
+#define TK 16
+#define TLD (TB + 2)  // padded LDS row (doubles) against bank conflicts
+
+__global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const double *__restrict__ Q,
+                                                      const double *__restrict__ lam,
+                                                      const int32_t *__restrict__ nkeep,
+                                                      const double *__restrict__ psi,
+                                                      double *__restrict__ Y, int N, int nk_max,
+                                                      int C, double dt) {
+    __shared__ double As[2][TK][TLD];  // [k][m]
+    __shared__ double Bs[2][TK][TLD];  // [k][n]
+    const int l = blockIdx.z;
+    const int nk = nkeep[l];
+    const int a0 = blockIdx.x * TB;
+    if (a0 >= nk) return;  // block-uniform
+    const int n0 = blockIdx.y * TB;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const double *Ql = Q + (size_t)l * N * nk_max;
+    const double *Xl = psi + (size_t)l * C * N * 2;
+    // global -> register mapping
+    const int ak = t >> 4, am = (t & 15) * 4;          // A: row k, 4 consecutive m
+    const int bc = t >> 3, bk = (t & 7) * 2;           // B: complex column, 2 consecutive k
+    const int cg = (n0 >> 1) + bc;
+    const bool cok = cg < C;
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+        const int i = k0 + ak;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            ra[u] = (i < N && a0 + am + u < nk_max) ? Ql[(size_t)i * nk_max + a0 + am + u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int ii = k0 + bk + u;
+            const bool ok = cok && ii < N;
+            rb[2 * u] = ok ? Xl[((size_t)cg * N + ii) * 2] : 0.0;
+            rb[2 * u + 1] = ok ? Xl[((size_t)cg * N + ii) * 2 + 1] : 0.0;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) As[buf][ak][am + u] = ra[u];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            Bs[buf][bk + u][2 * bc] = rb[2 * u];
+            Bs[buf][bk + u][2 * bc + 1] = rb[2 * u + 1];
+        }
+    };
+    d4_t acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    load(0);
+    stash(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < N; k0 += TK) {
+        const bool more = k0 + TK < N;
+        if (more) load(k0 + TK);
+#pragma unroll
+        for (int ks = 0; ks < TK / 4; ++ks) {
+            const int kk = ks * 4 + kr;
+            const double a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
+            const double b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
+            acc[0][0] = mfma64(a0v, b0v, acc[0][0]);
+            acc[0][1] = mfma64(a0v, b1v, acc[0][1]);
+            acc[1][0] = mfma64(a1v, b0v, acc[1][0]);
+            acc[1][1] = mfma64(a1v, b1v, acc[1][1]);
+        }
+        if (more) stash(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    const double *laml = lam + (size_t)l * nk_max;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int col = n0 + wn + 16 * y + lc;
+            const int c = col >> 1, p = col & 1;
+            const int ab = a0 + wm + 16 * x + 4 * kr;
+            double o[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = ab + r;
+                const double v = acc[x][y][r];
+                const double vo = __shfl_xor(v, 1);
+                const double re = p ? vo : v, im = p ? v : vo;
+                double sn, cs;
+                sincos(laml[a < nk ? a : 0] * dt, &sn, &cs);
+                o[r] = p ? (cs * im - sn * re) : (cs * re + sn * im);
+            }
+            if (c < C) {
+                double *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (ab + r < nk) Yo[ab + r] = o[r];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const double *__restrict__ Q,
+                                                      const int32_t *__restrict__ nkeep,
+                                                      const double *__restrict__ Y,
+                                                      double *__restrict__ psi, int N, int nk_max,
+                                                      int C) {
+    __shared__ double As[2][TK][TLD];  // [k = a][m = i]
+    __shared__ double Bs[2][TK][TLD];  // [k = a][n = col]
+    const int l = blockIdx.z;
+    const int nk = nkeep[l];
+    const int i0 = blockIdx.x * TB;
+    const int n0 = blockIdx.y * TB;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const double *Ql = Q + (size_t)l * N * nk_max;
+    const double *Yl = Y + (size_t)l * C * 2 * nk_max;
+    // A: Q[i0 + am][k0 + ak .. +4]  (row i, 4 consecutive a); B: Y[col][k0 + bk .. +4]
+    const int am = t >> 2, ak = (t & 3) * 4;
+    const int bn = t >> 2, bk = (t & 3) * 4;
+    const int ia = i0 + am;
+    const int colb = n0 + bn;
+    const bool bok = (colb >> 1) < C;
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int a = k0 + ak + u;
+            ra[u] = (ia < N && a < nk) ? Ql[(size_t)ia * nk_max + a] : 0.0;
+            const int ab = k0 + bk + u;
+            rb[u] = (bok && ab < nk) ? Yl[(size_t)colb * nk_max + ab] : 0.0;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            As[buf][ak + u][am] = ra[u];
+            Bs[buf][bk + u][bn] = rb[u];
+        }
+    };
+    d4_t acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    load(0);
+    stash(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < nk; k0 += TK) {
+        const bool more = k0 + TK < nk;
+        if (more) load(k0 + TK);
+#pragma unroll
+        for (int ks = 0; ks < TK / 4; ++ks) {
+            const int kk = ks * 4 + kr;
+            const double a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
+            const double b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
+            acc[0][0] = mfma64(a0v, b0v, acc[0][0]);
+            acc[0][1] = mfma64(a0v, b1v, acc[0][1]);
+            acc[1][0] = mfma64(a1v, b0v, acc[1][0]);
+            acc[1][1] = mfma64(a1v, b1v, acc[1][1]);
+        }
+        if (more) stash(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    double *Xl = psi + (size_t)l * C * N * 2;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int col = n0 + wn + 16 * y + lc;
+            const int c = col >> 1, p = col & 1;
+            if (c >= C) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wm + 16 * x + 4 * kr + r;
+                if (i < N) Xl[((size_t)c * N + i) * 2 + p] = acc[x][y][r];
+            }
+        }
     }
 }
 
@@ -858,7 +1053,15 @@ static int launch_first_half(ggb_ctx *ctx, int step) {
 static int launch_kinetic_pass(ggb_ctx *ctx, int pass) {
     const ggb_problem &p = ctx->p;
     const int ncol = (2 * ctx->C + 15) / 16;
-    if (pass == 0)
+    if (ctx->C > KIN_NARROW_MAX_C) {
+        const int nct = (2 * ctx->C + TB - 1) / TB;
+        if (pass == 0)
+            k_kin_fwd_tiled<<<dim3((p.nk_max + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
+                ctx->Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->Y, p.n_r, p.nk_max, ctx->C, p.dt);
+        else
+            k_kin_bwd_tiled<<<dim3((p.n_r + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
+                ctx->Q, ctx->nkeep, ctx->Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
+    } else if (pass == 0)
         k_kin_fwd<<<dim3((p.nk_max + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
             ctx->Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->Y, p.n_r, p.nk_max, ctx->C, p.dt);
     else

@@ -111,3 +111,29 @@ def test_short_run(atom, N, R, lmax, nsteps):
         assert rel < 1e-10, (key, rel)
     assert np.abs(out["norm"][:, 0, :] - ref["norm"]).max() < 1e-12
     assert np.abs(psi - ref["psi"]).max() < 1e-10
+
+
+def test_batched_instances_match_alone():
+    """Instances with different fields in one context give the same results
+    as each instance alone (per-column arithmetic does not depend on the mix)."""
+    _say("case: batched vs alone, Ne N=48 Lmax=6, 3 instances")
+    dt = 0.05
+    S = AtomSystem("Ne", 48, 30.0, 6, dt)
+    nsteps = 25
+    t = (np.arange(nsteps) + 0.5) * dt
+    fields = np.stack([0.05 * np.sin(0.4 * t), -0.08 * np.cos(0.3 * t), 0.02 + 0.0 * t])
+    eng = _engine()
+    with eng.Propagator(S, n_inst=3) as prop:
+        prop.set_field(fields)
+        mix = prop.run(0, nsteps)
+        psi_mix = prop.download_orbitals()
+    for k in range(3):
+        with eng.Propagator(S, n_inst=1) as prop:
+            prop.set_field(fields[k])
+            alone = prop.run(0, nsteps)
+            psi_alone = prop.download_orbitals()
+        for key in ("dip", "acc", "norm"):
+            d = np.abs(mix[key][:, k, :] - alone[key][:, 0, :]).max()
+            assert d <= 1e-14 * max(1.0, np.abs(alone[key]).max()), (k, key, d)
+        cols = slice(k * S.n_orb, (k + 1) * S.n_orb)
+        assert np.abs(psi_mix[:, cols, :] - psi_alone).max() < 1e-14
