@@ -808,6 +808,7 @@ struct ggb_ctx {
     std::string err;
     bool have_static, have_orbitals;
     int n_field;
+    bool field_ring;  // ggb_field_ring: field index = step mod n_field
     // device buffers
     double *psi, *Y, *Q, *lam, *B, *PL, *chi, *omega, *r, *wr, *v_gs, *mask, *occ, *rho_frozen;
     int32_t *nkeep, *mcol;
@@ -865,6 +866,32 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     delete ctx;
 }
 
+static size_t footprint_bytes(const ggb_problem *p, int Nlp, int Nthp, int ntiles) {
+    const size_t N = p->n_r, Nl = p->n_l, C = (size_t)p->n_orb * p->n_inst, Nth = p->n_theta;
+    const size_t nI = p->n_inst, nk = p->nk_max, nh = p->n_har;
+    size_t d = 0;  // doubles
+    d += Nl * C * N * 2;                 // psi
+    d += Nl * C * nk * 2;                // Y
+    d += Nl * N * nk + Nl * nk;          // Q, lam
+    d += (size_t)p->n_m * Nthp * Nlp;    // B
+    d += Nth * nh + 2 * Nth;             // PL, chi, omega
+    d += 5 * N + p->n_orb;               // r, wr, v_gs, mask, rho_frozen, occ
+    d += C * N * Nth;                    // dens
+    d += 2 * nI * N * Nth;               // vxc, dv
+    d += 2 * nI * N * Nth;               // phase (double2)
+    d += 2 * nI * nh * N;                // q, V
+    d += C * (size_t)ntiles * 3;         // obs_part
+    return d * sizeof(double) + (Nl + p->n_orb) * sizeof(int32_t);
+}
+
+extern "C" int ggb_footprint(const ggb_problem *p, uint64_t *bytes) {
+    if (!p || !bytes || p->n_r < 1 || p->n_l < 1 || p->n_theta < 1 || p->n_orb < 1 || p->n_inst < 1)
+        return GGB_E_ARG;
+    const int Nlp = ((p->n_l + 15) / 16) * 16, Nthp = ((p->n_theta + 15) / 16) * 16;
+    *bytes = footprint_bytes(p, Nlp, Nthp, (p->n_r + RT - 1) / RT);
+    return GGB_OK;
+}
+
 extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     if (!p || !out) {
         g_create_err = "ggb_create: null argument";
@@ -892,6 +919,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->ntiles = (p->n_r + RT - 1) / RT;
     ctx->have_static = ctx->have_orbitals = false;
     ctx->n_field = 0;
+    ctx->field_ring = false;
     ctx->obs_cap = 0;
     ctx->psi = ctx->Y = ctx->Q = ctx->lam = ctx->B = ctx->PL = ctx->chi = ctx->omega = nullptr;
     ctx->r = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
@@ -1110,6 +1138,33 @@ extern "C" int ggb_upload_orbitals(ggb_ctx *ctx, const double *psi) {
     return GGB_OK;
 }
 
+extern "C" int ggb_upload_columns(ggb_ctx *ctx, int32_t c0, int32_t nc, const double *psi) {
+    if (!ctx) return GGB_E_ARG;
+    if (!psi || c0 < 0 || nc < 1 || c0 + nc > ctx->C)
+        return fail(ctx, GGB_E_ARG, "ggb_upload_columns: bad column range");
+    if (!ctx->have_orbitals) return fail(ctx, GGB_E_STATE, "ggb_upload_columns: upload all orbitals first");
+    const ggb_problem &p = ctx->p;
+    HIPCHK(ctx, hipSetDevice(p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t row = (size_t)p.n_r * 16;  // one column of one l
+    HIPCHK(ctx, hipMemcpy2D(ctx->psi + (size_t)c0 * p.n_r * 2, ctx->C * row, psi, nc * row, nc * row,
+                            p.n_l, hipMemcpyHostToDevice));
+    return GGB_OK;
+}
+
+extern "C" int ggb_rebuild_potential(ggb_ctx *ctx) {
+    if (!ctx) return GGB_E_ARG;
+    if (!ctx->have_orbitals) return fail(ctx, GGB_E_STATE, "ggb_rebuild_potential: no orbitals");
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    const dim3 ga(ctx->ntiles, ctx->C);
+    k_angular<2><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, 0));
+    HIPCHK(ctx, hipGetLastError());
+    int rc = launch_potential(ctx);
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return GGB_OK;
+}
+
 extern "C" int ggb_set_field(ggb_ctx *ctx, const double *E, int32_t n_steps) {
     if (!ctx) return GGB_E_ARG;
     if (!E || n_steps < 1) return fail(ctx, GGB_E_ARG, "ggb_set_field: empty field");
@@ -1125,6 +1180,44 @@ extern "C" int ggb_set_field(ggb_ctx *ctx, const double *E, int32_t n_steps) {
                                ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->n_field = n_steps;
+    ctx->field_ring = false;
+    return GGB_OK;
+}
+
+extern "C" int ggb_field_ring(ggb_ctx *ctx, int32_t capacity) {
+    if (!ctx) return GGB_E_ARG;
+    if (capacity < 1) return fail(ctx, GGB_E_ARG, "ggb_field_ring: capacity must be positive");
+    const ggb_problem &p = ctx->p;
+    HIPCHK(ctx, hipSetDevice(p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->E) {
+        hipFree(ctx->E);
+        ctx->E = nullptr;
+    }
+    HIPCHK(ctx, dalloc(&ctx->E, (size_t)p.n_inst * capacity));
+    HIPCHK(ctx, hipMemset(ctx->E, 0, (size_t)p.n_inst * capacity * 8));
+    ctx->n_field = capacity;
+    ctx->field_ring = true;
+    return GGB_OK;
+}
+
+extern "C" int ggb_set_field_slice(ggb_ctx *ctx, int32_t inst, int32_t start, int32_t n,
+                                   const double *E) {
+    if (!ctx) return GGB_E_ARG;
+    const ggb_problem &p = ctx->p;
+    if (!E || inst < 0 || inst >= p.n_inst || start < 0 || n < 0)
+        return fail(ctx, GGB_E_ARG, "ggb_set_field_slice: bad instance or range");
+    if (!ctx->E || ctx->n_field < 1) return fail(ctx, GGB_E_STATE, "ggb_set_field_slice: no field buffer");
+    if (!ctx->field_ring && (long)start + n > ctx->n_field)
+        return fail(ctx, GGB_E_ARG, "ggb_set_field_slice: range outside the field");
+    if (n > ctx->n_field) return fail(ctx, GGB_E_ARG, "ggb_set_field_slice: longer than the ring");
+    HIPCHK(ctx, hipSetDevice(p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    double *base = ctx->E + (size_t)inst * ctx->n_field;
+    const int s0 = start % ctx->n_field;
+    const int first = (s0 + n <= ctx->n_field) ? n : ctx->n_field - s0;
+    HIPCHK(ctx, hipMemcpy(base + s0, E, (size_t)first * 8, hipMemcpyHostToDevice));
+    if (first < n) HIPCHK(ctx, hipMemcpy(base, E + first, (size_t)(n - first) * 8, hipMemcpyHostToDevice));
     return GGB_OK;
 }
 
@@ -1132,7 +1225,7 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
                        double *norm) {
     if (!ctx) return GGB_E_ARG;
     if (!ctx->have_orbitals) return fail(ctx, GGB_E_STATE, "ggb_run: orbitals not uploaded");
-    if (nsteps < 0 || step0 < 0 || (long)step0 + nsteps > ctx->n_field)
+    if (nsteps < 0 || step0 < 0 || (!ctx->field_ring && (long)step0 + nsteps > ctx->n_field))
         return fail(ctx, GGB_E_ARG, "ggb_run: steps outside the uploaded field");
     if (nsteps == 0) return GGB_OK;
     HIPCHK(ctx, hipSetDevice(ctx->p.device));
@@ -1160,7 +1253,7 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         return hipEventRecord(ctx->events[(size_t)s * ne + k], ctx->stream);
     };
     for (int s = 0; s < nsteps; ++s) {
-        const int step = step0 + s;
+        const int step = (int)(((long)step0 + s) % ctx->n_field);
         HIPCHK(ctx, mark(s, 0));
         int rc = launch_first_half(ctx, step);
         HIPCHK(ctx, mark(s, 1));
@@ -1225,7 +1318,9 @@ extern "C" int ggb_download_potential(ggb_ctx *ctx, double *dv) {
 extern "C" int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step) {
     if (!ctx) return GGB_E_ARG;
     if (!ctx->have_orbitals) return fail(ctx, GGB_E_STATE, "ggb_debug_substep: orbitals not uploaded");
-    if (step < 0 || step >= ctx->n_field) return fail(ctx, GGB_E_ARG, "ggb_debug_substep: step outside field");
+    if (step < 0 || (!ctx->field_ring && step >= ctx->n_field))
+        return fail(ctx, GGB_E_ARG, "ggb_debug_substep: step outside field");
+    step = step % ctx->n_field;
     HIPCHK(ctx, hipSetDevice(ctx->p.device));
     int rc;
     switch (which) {

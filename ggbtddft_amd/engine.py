@@ -11,6 +11,28 @@ import numpy as np
 from . import _lib
 
 
+def problem_for(system, n_inst=1, device=0):
+    """ggb_problem of a context holding n_inst instances of `system`."""
+    p = _lib.Problem()
+    p.abi_version = _lib.GGB_ABI_VERSION
+    p.n_r, p.n_l, p.n_theta = system.N, system.n_l, system.n_theta
+    p.n_har, p.n_orb, p.n_inst = system.n_har, system.n_orb, int(n_inst)
+    p.n_m, p.nk_max = system.angular.n_m, system.nk_max
+    p.interacting = 1 if system.interacting else 0
+    p.precision, p.device, p.dt = 0, int(device), system.dt
+    return p
+
+
+def footprint(system, n_inst=1, field_steps=0, obs_steps=0):
+    """Device bytes of a context (ggb_footprint) plus its field and observable buffers."""
+    lib = _lib.load()
+    b = ctypes.c_uint64()
+    p = problem_for(system, n_inst)
+    _lib.check(lib.ggb_footprint(ctypes.byref(p), ctypes.byref(b)), None, "ggb_footprint")
+    return int(b.value) + 8 * int(n_inst) * int(field_steps) + \
+        24 * system.n_orb * int(n_inst) * int(obs_steps)
+
+
 class Propagator:
     """Batched propagator for n_inst instances of one AtomSystem.
 
@@ -23,13 +45,7 @@ class Propagator:
         self.sys = system
         self.n_inst = int(n_inst)
         self.C = system.n_orb * self.n_inst
-        p = _lib.Problem()
-        p.abi_version = _lib.GGB_ABI_VERSION
-        p.n_r, p.n_l, p.n_theta = system.N, system.n_l, system.n_theta
-        p.n_har, p.n_orb, p.n_inst = system.n_har, system.n_orb, self.n_inst
-        p.n_m, p.nk_max = system.angular.n_m, system.nk_max
-        p.interacting = 1 if system.interacting else 0
-        p.precision, p.device, p.dt = 0, int(device), system.dt
+        p = problem_for(system, self.n_inst, device)
         self._problem = p
         ctx = ctypes.c_void_p()
         _lib.check(self.lib.ggb_create(ctypes.byref(p), ctypes.byref(ctx)), None, "ggb_create")
@@ -81,6 +97,29 @@ class Propagator:
         assert E.shape[0] == self.n_inst
         self._check(self.lib.ggb_set_field(self.ctx, _lib.dptr(E), E.shape[1]), "ggb_set_field")
         self.n_field = E.shape[1]
+
+    def field_ring(self, capacity):
+        """Zeroed ring field buffer: step n reads sample n mod capacity."""
+        self._check(self.lib.ggb_field_ring(self.ctx, int(capacity)), "ggb_field_ring")
+        self.n_field = int(capacity)
+
+    def set_field_slice(self, inst, start, E):
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        self._check(self.lib.ggb_set_field_slice(self.ctx, int(inst), int(start), E.size,
+                                                 _lib.dptr(E)), "ggb_set_field_slice")
+
+    def upload_instance(self, inst, psi=None):
+        """Seat orbitals [n_l, n_orb, N] (default: ground state) in instance slot
+        `inst` and rebuild the potentials (other instances are unchanged)."""
+        if psi is None:
+            psi = self.sys.psi0
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert psi.shape == (self.sys.n_l, self.sys.n_orb, self.sys.N)
+        c0 = int(inst) * self.sys.n_orb
+        self._check(self.lib.ggb_upload_columns(self.ctx, c0, self.sys.n_orb,
+                                                _lib.dptr(psi.view(np.float64))),
+                    "ggb_upload_columns")
+        self._check(self.lib.ggb_rebuild_potential(self.ctx), "ggb_rebuild_potential")
 
     def run(self, step0, nsteps, observables=True):
         """Propagate nsteps steps; returns dict(dip, acc, norm) of shape

@@ -82,6 +82,12 @@ typedef struct ggb_static {
 
 int ggb_abi_version(void);
 
+/* Device bytes a context of these sizes allocates, excluding the field and
+ * the per-call observable buffer (8 * n_inst * n_field and 24 * C * nsteps
+ * bytes); used for admission by memory footprint.  No counterpart in the
+ * reference. */
+int ggb_footprint(const ggb_problem *p, uint64_t *bytes);
+
 /* Create a context and allocate its device memory. */
 int ggb_create(const ggb_problem *p, ggb_ctx **out);
 
@@ -97,6 +103,28 @@ int ggb_upload_orbitals(ggb_ctx *ctx, const double *psi);
 /* Upload the field E[inst][step] for n_steps steps.  Replaces
  * h_initialiseField (h_initialiseField.cuh:1-40). */
 int ggb_set_field(ggb_ctx *ctx, const double *E, int32_t n_steps);
+
+/* Switch the field to a zeroed ring buffer of `capacity` steps per
+ * instance: step n then reads sample n mod capacity, so a long-running
+ * context can admit new instances at any step (continuous batching).
+ * No counterpart in the reference (its scans run one after another,
+ * main.cu:460-481). */
+int ggb_field_ring(ggb_ctx *ctx, int32_t capacity);
+
+/* Write n field samples of one instance starting at step `start` (wrapping
+ * in ring mode).  Per-scan counterpart of h_initialiseField. */
+int ggb_set_field_slice(ggb_ctx *ctx, int32_t inst, int32_t start, int32_t n, const double *E);
+
+/* Replace orbital columns c0 .. c0+nc-1 (psi[l][nc][i], complex interleaved)
+ * without touching the others, e.g. to seat a new instance in a free slot;
+ * follow with ggb_rebuild_potential.  Per-scan counterpart of
+ * h_initialiseTDKSOSet (main.cu:463). */
+int ggb_upload_columns(ggb_ctx *ctx, int32_t c0, int32_t nc, const double *psi);
+
+/* Rebuild the potential of every instance from the current orbitals.  For
+ * instances whose orbitals did not change this reproduces the potential
+ * they already hold, bit for bit. */
+int ggb_rebuild_potential(ggb_ctx *ctx);
 
 /* Propagate steps step0 .. step0+nsteps-1 and copy the per-step observables
  * <r cos theta>, <cos theta / r^2> and <psi|psi> of every orbital column into
