@@ -1,0 +1,70 @@
+"""Reference-format inputs/outputs: the init-key reader and the
+workingData/output writers."""
+import os
+
+import numpy as np
+import pytest
+
+from ggbtddft_amd import io
+from ggbtddft_amd.config import RunConfig, read_init
+from ggbtddft_amd.system import AtomSystem
+
+INIT = """# Target System
+atom = Xe
+N_r  = 512
+zeta = 1.0
+N_l = 256
+N_t = 16384
+dt  = 1.0
+I0 = 1.3E14
+omega0 = 0.0253
+tau0 = 2.0 * np.pi / 0.0253 / 41.3413745758
+CEP0 = 0.5 * np.pi
+IN_SITU = 1
+N_scan = 256
+USE_HARTREE = 1
+EXP_METHOD = CN
+"""
+
+
+def test_init_reader_matches_reference_conventions():
+    c = RunConfig(read_init(INIT))
+    assert c.atom == "Xe" and c.N == 512 and c.lmax == 255 and c.n_scan == 256
+    assert abs(c.dt - 0.041341374575751) < 1e-15          # 1 as in a.u.
+    assert abs(c.E0 - np.sqrt(1.3e14 / 3.509e16)) < 1e-15
+    assert abs(c.tau0 - 2 * np.pi / 0.0253) < 1e-9        # fs -> a.u. round trip
+    assert abs(c.r_max - 1.5 * c.E0 / 0.0253 ** 2) < 1e-9
+    t = c.time_grid()
+    assert t[0] == -0.5 * c.n_t * c.dt and abs(t[1] - t[0] - c.dt) < 1e-12
+
+
+def test_init_reader_refuses_code():
+    with pytest.raises(ValueError):
+        read_init("atom = __import__('os').system('true')")
+    with pytest.raises(ValueError):
+        read_init("N_r = open('x').read()")
+
+
+def test_working_data_round_trip(tmp_path):
+    S = AtomSystem("Ar", 48, 40.0, 6, 0.05)
+    io.write_working_data(str(tmp_path), S, np.arange(10) * 0.05)
+    psi = io.read_eigenstates(str(tmp_path), S.atom.target_id, S.n_l, S.n_orb, S.N)
+    assert np.array_equal(psi, S.psi0)
+    occ = np.fromfile(os.path.join(tmp_path, "stateParameters", "occ_active.bin"), dtype=np.int32)
+    assert list(occ) == [2, 2, 2, 2, 2, 4, 4]
+    r = np.fromfile(os.path.join(tmp_path, "mesh", "r.bin"))
+    assert np.array_equal(r, S.grid.r)
+
+
+def test_output_tree(tmp_path):
+    S = AtomSystem("He", 32, 30.0, 4, 0.05)
+    nt = 20
+    scans = [dict(E=np.ones(nt), dip=np.arange(nt, dtype=float)[:, None], acc_total=np.zeros(nt),
+                  omega=np.zeros(3), spectrum=np.ones(3))]
+    base = io.write_output(str(tmp_path), 7, S, np.arange(nt) * 0.05, scans)
+    d = np.fromfile(os.path.join(base, "timeDependent", "observables", "dipoleMoment", "scan000",
+                                 "kso000", "dipL000.bin"), dtype=np.complex128)
+    assert np.array_equal(d.real, np.arange(nt))
+    z = np.fromfile(os.path.join(base, "timeDependent", "observables", "dipoleMoment", "scan000",
+                                 "kso000", "dipL003.bin"), dtype=np.complex128)
+    assert z.size == nt and not z.any()
