@@ -68,7 +68,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel on MIB MiB (profiler byte-counter "
+                         "calibration) and print its bandwidth")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import ctypes
+        from ggbtddft_amd import _lib
+        ms = ctypes.c_double()
+        nb = args.copy_calibration << 20
+        _lib.check(_lib.load().ggb_selftest_copy(0, nb, ctypes.byref(ms)), None, "ggb_selftest_copy")
+        print(json.dumps({"copy_bytes": nb, "ms": ms.value,
+                          "GBps_read_plus_write": 2 * nb / (ms.value * 1e-3) / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs single-device instances "
               "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)

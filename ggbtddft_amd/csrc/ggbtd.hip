@@ -789,6 +789,13 @@ __global__ __launch_bounds__(WG) void k_dv(const double *__restrict__ V,
     }
 }
 
+// plain streaming copy of known size (calibrates the HBM byte counters)
+__global__ __launch_bounds__(WG) void k_copy(const double2 *__restrict__ src,
+                                             double2 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n; i += (size_t)gridDim.x * WG)
+        dst[i] = src[i];
+}
+
 __global__ void k_selftest_mfma(double *out) {
     const int lane = threadIdx.x;
     const double a = (double)(lane % 16) + 0.25 * (double)(lane / 16);
@@ -1377,4 +1384,35 @@ extern "C" int ggb_get_timing(const ggb_ctx *ctx, double *ms_total, int64_t *cou
         count[k] = ctx->phase_n[k];
     }
     return GGB_OK;
+}
+
+extern "C" int ggb_selftest_copy(int32_t device, uint64_t bytes, double *ms) {
+    if (bytes < 16 || bytes % 16 != 0 || !ms) return GGB_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return GGB_E_HIP;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return GGB_E_NOMEM;
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return GGB_E_NOMEM;
+    }
+    hipError_t e = hipMemset(a, 0, bytes);
+    hipEvent_t e0, e1;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const size_t n = bytes / 16;
+    if (e == hipSuccess) {
+        k_copy<<<2048, WG>>>((const double2 *)a, (double2 *)b, n);  // warm
+        (void)hipEventRecord(e0, 0);
+        k_copy<<<2048, WG>>>((const double2 *)a, (double2 *)b, n);
+        (void)hipEventRecord(e1, 0);
+        e = hipEventSynchronize(e1);
+        float f = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+        *ms = f;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return e == hipSuccess ? GGB_OK : GGB_E_HIP;
 }

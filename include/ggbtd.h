@@ -165,6 +165,11 @@ int ggb_get_timing(const ggb_ctx *ctx, double *ms_total, int64_t *count);
  * B[k][j] = 0.5 k - j, written to out[16][16] (lane-layout self test). */
 int ggb_selftest_mfma(int32_t device, double *out);
 
+/* Copy `bytes` (multiple of 16) device-to-device with a plain streaming
+ * kernel (k_copy) twice and report the second launch's time: the known-size
+ * kernel against which the profiler's HBM byte counters are calibrated. */
+int ggb_selftest_copy(int32_t device, uint64_t bytes, double *ms);
+
 /* Message of the last failure on ctx (or of the last failed ggb_create when
  * ctx is NULL). */
 const char *ggb_last_error(const ggb_ctx *ctx);
