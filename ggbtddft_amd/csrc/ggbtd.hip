@@ -50,6 +50,22 @@ static __device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// Matrix-core product D += A B of one 16x16x4 tile in the compute type T
+// (FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32; same lane layout).
+template <typename T> struct mfma_t;
+template <> struct mfma_t<double> {
+    typedef d4_t v4;
+    static __device__ __forceinline__ v4 op(double a, double b, v4 c) { return mfma64(a, b, c); }
+};
+template <> struct mfma_t<float> {
+    typedef f4_t v4;
+    static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
 static __device__ __forceinline__ double c_lm(int l, int m) {
     // <l m|cos theta|l+1 m> (internal.py:80-84, angularCoefficients.cuh d_c)
     double dl = (double)l, dm = (double)m;
@@ -60,15 +76,18 @@ static __device__ __forceinline__ double c_lm(int l, int m) {
 // Kinetic propagator, pass 1: Y[l][c][p][a] = (e^{-i lam_a dt} sum_i Q[l][i][a] psi[l][c][i])_p
 // M = eigen index a, N = real column (c, p = re/im), K = radial node i.
 // Y is planar in (re, im) so that pass 2 reads contiguous runs of a.
-// Each lane keeps KU loads of A and B in flight before the MFMAs.
+// Each lane keeps KU loads of A and B in flight before the MFMAs.  T is the
+// compute type (tables and Y in T, state in FP64, phase in FP64).
 // ---------------------------------------------------------------------------
 #define KU 16
-__global__ __launch_bounds__(WG) void k_kin_fwd(const double *__restrict__ Q,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 const double *__restrict__ lam,
                                                 const int32_t *__restrict__ nkeep,
                                                 const double *__restrict__ psi,
-                                                double *__restrict__ Y, int N, int nk_max, int C,
+                                                T *__restrict__ Y, int N, int nk_max, int C,
                                                 double dt) {
+    typedef typename mfma_t<T>::v4 v4;
     const int l = blockIdx.z;
     const int nk = nkeep[l];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -78,35 +97,35 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const double *__restrict__ Q,
     const int col = blockIdx.y * 16 + lc;
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
-    const double *Ql = Q + (size_t)l * N * nk_max + a0 + lc;
+    const T *Ql = Q + (size_t)l * N * nk_max + a0 + lc;
     const double *Xl = psi + ((size_t)l * C + (colok ? c : 0)) * N * 2 + p;
-    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+    v4 acc = {0, 0, 0, 0};
     for (int k0 = 0; k0 < N; k0 += 4 * KU) {
-        double av[KU], bv[KU];
+        T av[KU], bv[KU];
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
             const int i = k0 + 4 * u + kr;
             const bool ok = i < N;
-            av[u] = ok ? Ql[(size_t)i * nk_max] : 0.0;
-            bv[u] = (ok && colok) ? Xl[(size_t)i * 2] : 0.0;
+            av[u] = ok ? Ql[(size_t)i * nk_max] : (T)0;
+            bv[u] = (ok && colok) ? (T)Xl[(size_t)i * 2] : (T)0;
         }
 #pragma unroll
-        for (int u = 0; u < KU; ++u) acc = mfma64(av[u], bv[u], acc);
+        for (int u = 0; u < KU; ++u) acc = mfma_t<T>::op(av[u], bv[u], acc);
     }
     const double *laml = lam + (size_t)l * nk_max;
-    double o[4];
+    T o[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int a = a0 + 4 * kr + r;
-        const double v = acc[r];
+        const double v = (double)acc[r];
         const double vo = __shfl_xor(v, 1);
         const double re = p ? vo : v, im = p ? v : vo;
         double sn, cs;
         sincos(laml[a < nk ? a : 0] * dt, &sn, &cs);
-        o[r] = p ? (cs * im - sn * re) : (cs * re + sn * im);
+        o[r] = (T)(p ? (cs * im - sn * re) : (cs * re + sn * im));
     }
     if (colok) {
-        double *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max + a0 + 4 * kr;
+        T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max + a0 + 4 * kr;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             if (a0 + 4 * kr + r < nk) Yo[r] = o[r];
@@ -117,11 +136,13 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const double *__restrict__ Q,
 // K order is permuted (lane kr of MFMA j in chunk u takes a = k0 + 16u + 4kr + j)
 // so that each lane reads 4 consecutive a of Q and of planar Y.
 #define KU2 4
-__global__ __launch_bounds__(WG) void k_kin_bwd(const double *__restrict__ Q,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
                                                 const int32_t *__restrict__ nkeep,
-                                                const double *__restrict__ Y,
+                                                const T *__restrict__ Y,
                                                 double *__restrict__ psi, int N, int nk_max,
                                                 int C) {
+    typedef typename mfma_t<T>::v4 v4;
     const int l = blockIdx.z;
     const int nk = nkeep[l];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -133,29 +154,29 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const double *__restrict__ Q,
     const bool colok = c < C;
     const int im_ = i0 + lc;
     const bool rowok = im_ < N;
-    const double *Ql = Q + (size_t)l * N * nk_max + (size_t)(rowok ? im_ : 0) * nk_max;
-    const double *Yl = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max;
-    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+    const T *Ql = Q + (size_t)l * N * nk_max + (size_t)(rowok ? im_ : 0) * nk_max;
+    const T *Yl = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max;
+    v4 acc = {0, 0, 0, 0};
     for (int k0 = 0; k0 < nk; k0 += 16 * KU2) {
-        double av[4 * KU2], bv[4 * KU2];
+        T av[4 * KU2], bv[4 * KU2];
 #pragma unroll
         for (int u = 0; u < KU2; ++u) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int a = k0 + 16 * u + 4 * kr + j;
                 const bool ok = a < nk;
-                av[4 * u + j] = (ok && rowok) ? Ql[a] : 0.0;
-                bv[4 * u + j] = (ok && colok) ? Yl[a] : 0.0;
+                av[4 * u + j] = (ok && rowok) ? Ql[a] : (T)0;
+                bv[4 * u + j] = (ok && colok) ? Yl[a] : (T)0;
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4 * KU2; ++u) acc = mfma64(av[u], bv[u], acc);
+        for (int u = 0; u < 4 * KU2; ++u) acc = mfma_t<T>::op(av[u], bv[u], acc);
     }
     double *Xl = psi + (size_t)l * C * N * 2;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = i0 + 4 * kr + r;
-        if (colok && i < N) Xl[((size_t)c * N + i) * 2 + p] = acc[r];
+        if (colok && i < N) Xl[((size_t)c * N + i) * 2 + p] = (double)acc[r];
     }
 }
 
@@ -168,16 +189,18 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const double *__restrict__ Q,
 // ---------------------------------------------------------------------------
 #define TB 64
 #define TK 16
-#define TLD (TB + 2)  // padded LDS row (doubles) against bank conflicts
+#define TLD (TB + 2)  // padded LDS row against bank conflicts
 
-__global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const double *__restrict__ Q,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
                                                       const double *__restrict__ lam,
                                                       const int32_t *__restrict__ nkeep,
                                                       const double *__restrict__ psi,
-                                                      double *__restrict__ Y, int N, int nk_max,
+                                                      T *__restrict__ Y, int N, int nk_max,
                                                       int C, double dt) {
-    __shared__ double As[2][TK][TLD];  // [k][m]
-    __shared__ double Bs[2][TK][TLD];  // [k][n]
+    typedef typename mfma_t<T>::v4 v4;
+    __shared__ T As[2][TK][TLD];  // [k][m]
+    __shared__ T Bs[2][TK][TLD];  // [k][n]
     const int l = blockIdx.z;
     const int nk = nkeep[l];
     const int a0 = blockIdx.x * TB;
@@ -186,25 +209,25 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const double *__restrict__
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const double *Ql = Q + (size_t)l * N * nk_max;
+    const T *Ql = Q + (size_t)l * N * nk_max;
     const double *Xl = psi + (size_t)l * C * N * 2;
     // global -> register mapping
     const int ak = t >> 4, am = (t & 15) * 4;          // A: row k, 4 consecutive m
     const int bc = t >> 3, bk = (t & 7) * 2;           // B: complex column, 2 consecutive k
     const int cg = (n0 >> 1) + bc;
     const bool cok = cg < C;
-    double ra[4], rb[4];
+    T ra[4], rb[4];
     auto load = [&](int k0) {
         const int i = k0 + ak;
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            ra[u] = (i < N && a0 + am + u < nk_max) ? Ql[(size_t)i * nk_max + a0 + am + u] : 0.0;
+            ra[u] = (i < N && a0 + am + u < nk_max) ? Ql[(size_t)i * nk_max + a0 + am + u] : (T)0;
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int ii = k0 + bk + u;
             const bool ok = cok && ii < N;
-            rb[2 * u] = ok ? Xl[((size_t)cg * N + ii) * 2] : 0.0;
-            rb[2 * u + 1] = ok ? Xl[((size_t)cg * N + ii) * 2 + 1] : 0.0;
+            rb[2 * u] = ok ? (T)Xl[((size_t)cg * N + ii) * 2] : (T)0;
+            rb[2 * u + 1] = ok ? (T)Xl[((size_t)cg * N + ii) * 2 + 1] : (T)0;
         }
     };
     auto stash = [&](int buf) {
@@ -216,11 +239,11 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const double *__restrict__
             Bs[buf][bk + u][2 * bc + 1] = rb[2 * u + 1];
         }
     };
-    d4_t acc[2][2];
+    v4 acc[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < 2; ++y) acc[x][y] = (v4){0, 0, 0, 0};
     load(0);
     stash(0);
     __syncthreads();
@@ -231,12 +254,12 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const double *__restrict__
 #pragma unroll
         for (int ks = 0; ks < TK / 4; ++ks) {
             const int kk = ks * 4 + kr;
-            const double a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
-            const double b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
-            acc[0][0] = mfma64(a0v, b0v, acc[0][0]);
-            acc[0][1] = mfma64(a0v, b1v, acc[0][1]);
-            acc[1][0] = mfma64(a1v, b0v, acc[1][0]);
-            acc[1][1] = mfma64(a1v, b1v, acc[1][1]);
+            const T a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
+            const T b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
+            acc[0][0] = mfma_t<T>::op(a0v, b0v, acc[0][0]);
+            acc[0][1] = mfma_t<T>::op(a0v, b1v, acc[0][1]);
+            acc[1][0] = mfma_t<T>::op(a1v, b0v, acc[1][0]);
+            acc[1][1] = mfma_t<T>::op(a1v, b1v, acc[1][1]);
         }
         if (more) stash(buf ^ 1);
         __syncthreads();
@@ -250,19 +273,19 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const double *__restrict__
             const int col = n0 + wn + 16 * y + lc;
             const int c = col >> 1, p = col & 1;
             const int ab = a0 + wm + 16 * x + 4 * kr;
-            double o[4];
+            T o[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int a = ab + r;
-                const double v = acc[x][y][r];
+                const double v = (double)acc[x][y][r];
                 const double vo = __shfl_xor(v, 1);
                 const double re = p ? vo : v, im = p ? v : vo;
                 double sn, cs;
                 sincos(laml[a < nk ? a : 0] * dt, &sn, &cs);
-                o[r] = p ? (cs * im - sn * re) : (cs * re + sn * im);
+                o[r] = (T)(p ? (cs * im - sn * re) : (cs * re + sn * im));
             }
             if (c < C) {
-                double *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max;
+                T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max;
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     if (ab + r < nk) Yo[ab + r] = o[r];
@@ -271,13 +294,15 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const double *__restrict__
     }
 }
 
-__global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const double *__restrict__ Q,
+template <typename T>
+__global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
                                                       const int32_t *__restrict__ nkeep,
-                                                      const double *__restrict__ Y,
+                                                      const T *__restrict__ Y,
                                                       double *__restrict__ psi, int N, int nk_max,
                                                       int C) {
-    __shared__ double As[2][TK][TLD];  // [k = a][m = i]
-    __shared__ double Bs[2][TK][TLD];  // [k = a][n = col]
+    typedef typename mfma_t<T>::v4 v4;
+    __shared__ T As[2][TK][TLD];  // [k = a][m = i]
+    __shared__ T Bs[2][TK][TLD];  // [k = a][n = col]
     const int l = blockIdx.z;
     const int nk = nkeep[l];
     const int i0 = blockIdx.x * TB;
@@ -285,22 +310,22 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const double *__restrict__
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const double *Ql = Q + (size_t)l * N * nk_max;
-    const double *Yl = Y + (size_t)l * C * 2 * nk_max;
+    const T *Ql = Q + (size_t)l * N * nk_max;
+    const T *Yl = Y + (size_t)l * C * 2 * nk_max;
     // A: Q[i0 + am][k0 + ak .. +4]  (row i, 4 consecutive a); B: Y[col][k0 + bk .. +4]
     const int am = t >> 2, ak = (t & 3) * 4;
     const int bn = t >> 2, bk = (t & 3) * 4;
     const int ia = i0 + am;
     const int colb = n0 + bn;
     const bool bok = (colb >> 1) < C;
-    double ra[4], rb[4];
+    T ra[4], rb[4];
     auto load = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int a = k0 + ak + u;
-            ra[u] = (ia < N && a < nk) ? Ql[(size_t)ia * nk_max + a] : 0.0;
+            ra[u] = (ia < N && a < nk) ? Ql[(size_t)ia * nk_max + a] : (T)0;
             const int ab = k0 + bk + u;
-            rb[u] = (bok && ab < nk) ? Yl[(size_t)colb * nk_max + ab] : 0.0;
+            rb[u] = (bok && ab < nk) ? Yl[(size_t)colb * nk_max + ab] : (T)0;
         }
     };
     auto stash = [&](int buf) {
@@ -310,11 +335,11 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const double *__restrict__
             Bs[buf][bk + u][bn] = rb[u];
         }
     };
-    d4_t acc[2][2];
+    v4 acc[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < 2; ++y) acc[x][y] = (v4){0, 0, 0, 0};
     load(0);
     stash(0);
     __syncthreads();
@@ -325,12 +350,12 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const double *__restrict__
 #pragma unroll
         for (int ks = 0; ks < TK / 4; ++ks) {
             const int kk = ks * 4 + kr;
-            const double a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
-            const double b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
-            acc[0][0] = mfma64(a0v, b0v, acc[0][0]);
-            acc[0][1] = mfma64(a0v, b1v, acc[0][1]);
-            acc[1][0] = mfma64(a1v, b0v, acc[1][0]);
-            acc[1][1] = mfma64(a1v, b1v, acc[1][1]);
+            const T a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
+            const T b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
+            acc[0][0] = mfma_t<T>::op(a0v, b0v, acc[0][0]);
+            acc[0][1] = mfma_t<T>::op(a0v, b1v, acc[0][1]);
+            acc[1][0] = mfma_t<T>::op(a1v, b0v, acc[1][0]);
+            acc[1][1] = mfma_t<T>::op(a1v, b1v, acc[1][1]);
         }
         if (more) stash(buf ^ 1);
         __syncthreads();
@@ -347,7 +372,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const double *__restrict__
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = i0 + wm + 16 * x + 4 * kr + r;
-                if (i < N) Xl[((size_t)c * N + i) * 2 + p] = acc[x][y][r];
+                if (i < N) Xl[((size_t)c * N + i) * 2 + p] = (double)acc[x][y][r];
             }
         }
     }
@@ -356,14 +381,15 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const double *__restrict__
 // ---------------------------------------------------------------------------
 // Angular kernel.  One work-group = one orbital column c, RT radial nodes.
 // LDS holds the l-space tile X[Nlp][16] and the theta-space tile P[Nthp][16]
-// (real column n = 2 * i_local + re/im).
+// (real column n = 2 * i_local + re/im) in the compute type T; the state in
+// HBM, the phase factors and the observable sums stay FP64.
 // mode 0: Vh, Le, Lo          (first half of the step)
 // mode 1: Lo, Le, Vh, store, then |B psi|^2 + observable partials
 // mode 2: |B psi|^2 + observable partials only (initial potential)
 // ---------------------------------------------------------------------------
 struct AngArgs {
     double *psi;
-    const double *B;       // [n_m][Nthp][Nlp] zero padded
+    const void *B;         // [n_m][Nthp][Nlp] zero padded, compute type
     const int32_t *mcol;   // [n_orb]
     const double2 *phase;  // [n_inst][N][Nth] mask e^{-i dV dt/2}
     const double *r;
@@ -375,7 +401,8 @@ struct AngArgs {
     double dt;
 };
 
-static __device__ void ang_laser(double *X, int parity, int m, int Nl, const double *a_i) {
+template <typename T>
+static __device__ void ang_laser(T *X, int parity, int m, int Nl, const double *a_i) {
     // pairs (l, l+1), l = m + parity + 2 q, l + 1 < Nl; thread -> (q, i_local)
     const int t = threadIdx.x;
     const int il = t & 7;
@@ -383,11 +410,12 @@ static __device__ void ang_laser(double *X, int parity, int m, int Nl, const dou
         const int l = m + parity + 2 * q;
         if (l + 1 >= Nl) break;
         const double th = a_i[il] * c_lm(l, m);
-        double sn, cs;
-        sincos(th, &sn, &cs);
-        double *x0 = X + l * 16 + 2 * il;
-        double *x1 = X + (l + 1) * 16 + 2 * il;
-        const double r0 = x0[0], i0 = x0[1], r1 = x1[0], i1 = x1[1];
+        double snd, csd;
+        sincos(th, &snd, &csd);
+        const T sn = (T)snd, cs = (T)csd;
+        T *x0 = X + l * 16 + 2 * il;
+        T *x1 = X + (l + 1) * 16 + 2 * il;
+        const T r0 = x0[0], i0 = x0[1], r1 = x1[0], i1 = x1[1];
         // [x0', x1'] = [[cs, -i sn], [-i sn, cs]] [x0, x1]
         x0[0] = cs * r0 + sn * i1;
         x0[1] = cs * i0 - sn * r1;
@@ -399,27 +427,30 @@ static __device__ void ang_laser(double *X, int parity, int m, int Nl, const dou
 // theta-space product P = B X; accumulators for up to 4 M-tiles per wave.
 // K order permuted (lane kr of MFMA j takes l = k0 + 4kr + j) so that each
 // lane reads 4 consecutive l of its row of B; 8 loads in flight per lane.
-static __device__ void ang_fwd(const double *Bm, const double *X, int Nlp, int Nthp, d4_t *acc) {
+template <typename T>
+static __device__ void ang_fwd(const T *Bm, const T *X, int Nlp, int Nthp,
+                               typename mfma_t<T>::v4 *acc) {
+    typedef typename mfma_t<T>::v4 v4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int mt = Nthp / 16;
     for (int t = 0; t < 4; ++t) {
-        acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        acc[t] = (v4){0, 0, 0, 0};
         const int tile = wave + 4 * t;
         if (tile >= mt) continue;
-        const double *Arow = Bm + (size_t)(tile * 16 + lc) * Nlp + 4 * kr;
-        d4_t a = acc[t];
+        const T *Arow = Bm + (size_t)(tile * 16 + lc) * Nlp + 4 * kr;
+        v4 a = acc[t];
         for (int k0 = 0; k0 < Nlp; k0 += 32) {
-            double av[8];
+            T av[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int kb = k0 + 16 * (u >> 2) + (u & 3);
-                av[u] = (kb < Nlp) ? Arow[kb] : 0.0;
+                av[u] = (kb < Nlp) ? Arow[kb] : (T)0;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int kb = k0 + 16 * (u >> 2) + (u & 3);
-                if (kb < Nlp) a = mfma64(av[u], X[(kb + 4 * kr) * 16 + lc], a);
+                if (kb < Nlp) a = mfma_t<T>::op(av[u], X[(kb + 4 * kr) * 16 + lc], a);
             }
         }
         acc[t] = a;
@@ -427,39 +458,43 @@ static __device__ void ang_fwd(const double *Bm, const double *X, int Nlp, int N
 }
 
 // l-space product X = B^T P; up to 2 M-tiles per wave, 8 loads in flight.
-static __device__ void ang_bwd(const double *Bm, const double *P, int Nlp, int Nthp, d4_t *acc) {
+template <typename T>
+static __device__ void ang_bwd(const T *Bm, const T *P, int Nlp, int Nthp,
+                               typename mfma_t<T>::v4 *acc) {
+    typedef typename mfma_t<T>::v4 v4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int mt = Nlp / 16;
     for (int t = 0; t < 2; ++t) {
-        acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        acc[t] = (v4){0, 0, 0, 0};
         const int tile = wave + 4 * t;
         if (tile >= mt) continue;
-        const double *Acol = Bm + (size_t)kr * Nlp + tile * 16 + lc;
-        d4_t a = acc[t];
+        const T *Acol = Bm + (size_t)kr * Nlp + tile * 16 + lc;
+        v4 a = acc[t];
         for (int k0 = 0; k0 < Nthp; k0 += 32) {
-            double av[8];
+            T av[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int k = k0 + 4 * u;
-                av[u] = (k < Nthp) ? Acol[(size_t)k * Nlp] : 0.0;
+                av[u] = (k < Nthp) ? Acol[(size_t)k * Nlp] : (T)0;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int k = k0 + 4 * u;
-                if (k < Nthp) a = mfma64(av[u], P[(k + kr) * 16 + lc], a);
+                if (k < Nthp) a = mfma_t<T>::op(av[u], P[(k + kr) * 16 + lc], a);
             }
         }
         acc[t] = a;
     }
 }
 
-template <int MODE>
+template <int MODE, typename T>
 __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
+    typedef typename mfma_t<T>::v4 v4;
     extern __shared__ double lds[];
     const int Nlp = g.Nlp, Nthp = g.Nthp;
-    double *X = lds;                // [Nlp][16]
-    double *P = lds + Nlp * 16;     // [Nthp][16]
+    T *X = reinterpret_cast<T *>(lds);  // [Nlp][16]
+    T *P = X + Nlp * 16;                // [Nthp][16]
     __shared__ double a_i[RT];
     __shared__ double red[3][WG / 64];
 
@@ -467,7 +502,7 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
     const int i0 = tile_i * RT;
     const int inst = c / g.n_orb;
     const int m = g.mcol[c % g.n_orb];
-    const double *Bm = g.B + (size_t)m * Nthp * Nlp;
+    const T *Bm = reinterpret_cast<const T *>(g.B) + (size_t)m * Nthp * Nlp;
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
@@ -479,8 +514,8 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
         const int i = i0 + il;
         double2 v = make_double2(0.0, 0.0);
         if (l < g.Nl && i < g.N) v = ((const double2 *)g.psi)[((size_t)l * g.C + c) * g.N + i];
-        X[l * 16 + 2 * il] = v.x;
-        X[l * 16 + 2 * il + 1] = v.y;
+        X[l * 16 + 2 * il] = (T)v.x;
+        X[l * 16 + 2 * il + 1] = (T)v.y;
     }
     if (MODE != 2 && t < RT) {
         const int i = i0 + t;
@@ -490,16 +525,16 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
     __syncthreads();
 
     if (MODE == 1) {
-        ang_laser(X, 1, m, g.Nl, a_i);
+        ang_laser<T>(X, 1, m, g.Nl, a_i);
         __syncthreads();
-        ang_laser(X, 0, m, g.Nl, a_i);
+        ang_laser<T>(X, 0, m, g.Nl, a_i);
         __syncthreads();
     }
 
-    d4_t acc[4];
+    v4 acc[4];
     if (MODE != 2) {
         // Vh: theta space, phase, back to l space
-        ang_fwd(Bm, X, Nlp, Nthp, acc);
+        ang_fwd<T>(Bm, X, Nlp, Nthp, acc);
         const double2 *ph = g.phase + (size_t)inst * g.N * g.Nth;
         const int i = i0 + ilc;
         for (int tt = 0; tt < 4; ++tt) {
@@ -508,19 +543,20 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int j = tile * 16 + 4 * kr + r;
-                const double v = acc[tt][r];
-                const double vo = __shfl_xor(v, 1);
-                double o = 0.0;
+                const T v = acc[tt][r];
+                const T vo = __shfl_xor(v, 1);
+                T o = (T)0;
                 if (j < g.Nth && i < g.N) {
                     const double2 f = ph[(size_t)i * g.Nth + j];
-                    const double re = p ? vo : v, im = p ? v : vo;
-                    o = p ? (re * f.y + im * f.x) : (re * f.x - im * f.y);
+                    const T fx = (T)f.x, fy = (T)f.y;
+                    const T re = p ? vo : v, im = p ? v : vo;
+                    o = p ? (re * fy + im * fx) : (re * fx - im * fy);
                 }
                 P[j * 16 + lc] = o;
             }
         }
         __syncthreads();
-        ang_bwd(Bm, P, Nlp, Nthp, acc);
+        ang_bwd<T>(Bm, P, Nlp, Nthp, acc);
         for (int tt = 0; tt < 2; ++tt) {
             const int tile = wave + 4 * tt;
             if (tile >= Nlp / 16) continue;
@@ -531,9 +567,9 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
     }
 
     if (MODE == 0) {
-        ang_laser(X, 0, m, g.Nl, a_i);
+        ang_laser<T>(X, 0, m, g.Nl, a_i);
         __syncthreads();
-        ang_laser(X, 1, m, g.Nl, a_i);
+        ang_laser<T>(X, 1, m, g.Nl, a_i);
         __syncthreads();
     }
 
@@ -543,13 +579,13 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
             const int i = i0 + il;
             if (i < g.N)
                 ((double2 *)g.psi)[((size_t)l * g.C + c) * g.N + i] =
-                    make_double2(X[l * 16 + 2 * il], X[l * 16 + 2 * il + 1]);
+                    make_double2((double)X[l * 16 + 2 * il], (double)X[l * 16 + 2 * il + 1]);
         }
     }
     if (MODE == 0) return;
 
-    // |B psi|^2 and observable partials
-    ang_fwd(Bm, X, Nlp, Nthp, acc);
+    // |B psi|^2 and observable partials (accumulated in FP64)
+    ang_fwd<T>(Bm, X, Nlp, Nthp, acc);
     const int i = i0 + ilc;
     const bool iok = i < g.N;
     const double ri = iok ? g.r[i] : 1.0;
@@ -561,7 +597,7 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int j = tile * 16 + 4 * kr + r;
-            const double v = acc[tt][r];
+            const double v = (double)acc[tt][r];
             const double s2 = v * v + __shfl_xor(v * v, 1);
             if (p == 0 && iok && j < g.Nth) {
                 dens[(size_t)i * g.Nth + j] = s2;
@@ -605,22 +641,40 @@ __global__ void k_obs_reduce(const double *__restrict__ obs_part, double *__rest
 // ---------------------------------------------------------------------------
 // Density, LDA v_xc and multipole moments for one instance and RT radial nodes.
 // ---------------------------------------------------------------------------
-static __device__ double lda_vxc(double rho) {
-    // Slater exchange + Perdew-Wang 1992 correlation, spin unpolarised
-    if (!(rho > RHO_MIN)) return 0.0;
-    const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
-    const double vx = -cbrt(3.0 * rho / M_PI);
-    const double rs = cbrt(3.0 / (4.0 * M_PI * rho));
-    const double srs = sqrt(rs);
-    const double q0 = -2.0 * A * (1.0 + a1 * rs);
-    const double q1 = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
-    const double dq1 = A * (b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 4.0 * b4 * rs);
-    const double lg = log1p(1.0 / q1);
-    const double eps = q0 * lg;
-    const double deps = -2.0 * A * a1 * lg - q0 * dq1 / (q1 * q1 + q1);
-    return vx + eps - rs / 3.0 * deps;
+template <typename T> struct lda_t;
+template <> struct lda_t<double> {
+    static __device__ double cbrt_(double x) { return cbrt(x); }
+    static __device__ double sqrt_(double x) { return sqrt(x); }
+    static __device__ double log1p_(double x) { return log1p(x); }
+    static constexpr double rho_min = RHO_MIN;
+};
+template <> struct lda_t<float> {
+    static __device__ float cbrt_(float x) { return cbrtf(x); }
+    static __device__ float sqrt_(float x) { return sqrtf(x); }
+    static __device__ float log1p_(float x) { return log1pf(x); }
+    static constexpr float rho_min = 1e-30f;  // FP32: dropped exchange below 1e-10 Hartree
+};
+
+// Slater exchange + Perdew-Wang 1992 correlation, spin unpolarised, in T
+template <typename T>
+static __device__ T lda_vxc(T rho) {
+    typedef lda_t<T> F;
+    if (!(rho > (T)F::rho_min)) return (T)0;
+    const T A = (T)0.031091, a1 = (T)0.21370, b1 = (T)7.5957, b2 = (T)3.5876, b3 = (T)1.6382,
+            b4 = (T)0.49294, pi = (T)M_PI;
+    const T vx = -F::cbrt_((T)3 * rho / pi);
+    const T rs = F::cbrt_((T)3 / ((T)4 * pi * rho));
+    const T srs = F::sqrt_(rs);
+    const T q0 = (T)-2 * A * ((T)1 + a1 * rs);
+    const T q1 = (T)2 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+    const T dq1 = A * (b1 / srs + (T)2 * b2 + (T)3 * b3 * srs + (T)4 * b4 * rs);
+    const T lg = F::log1p_((T)1 / q1);
+    const T eps = q0 * lg;
+    const T deps = (T)-2 * A * a1 * lg - q0 * dq1 / (q1 * q1 + q1);
+    return vx + eps - rs / (T)3 * deps;
 }
 
+template <typename T>
 __global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
                                                 const double *__restrict__ occ,
                                                 const double *__restrict__ rho_frozen,
@@ -642,7 +696,7 @@ __global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
                 ru += occ[k] * dens[((size_t)(inst * n_orb + k) * N + i) * Nth + j];
             if (rho_frozen) ru += 0.5 * rho_frozen[i] * omega[j];
             const double rho = ru / (2.0 * M_PI * omega[j] * wr[i] * r[i] * r[i]);
-            vxc[((size_t)inst * N + i) * Nth + j] = lda_vxc(rho);
+            vxc[((size_t)inst * N + i) * Nth + j] = (double)lda_vxc<T>((T)rho);
         }
         rho_s[idx] = ru;
     }
@@ -803,6 +857,9 @@ __global__ void k_selftest_mfma(double *out) {
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
     acc = mfma64(a, b, acc);
     for (int r = 0; r < 4; ++r) out[(4 * (lane / 16) + r) * 16 + lane % 16] = acc[r];
+    f4_t accf = {0.f, 0.f, 0.f, 0.f};
+    accf = mfma_t<float>::op((float)a, (float)b, accf);
+    for (int r = 0; r < 4; ++r) out[256 + (4 * (lane / 16) + r) * 16 + lane % 16] = (double)accf[r];
 }
 
 // ===========================================================================
@@ -878,9 +935,9 @@ static size_t footprint_bytes(const ggb_problem *p, int Nlp, int Nthp, int ntile
     const size_t nI = p->n_inst, nk = p->nk_max, nh = p->n_har;
     size_t d = 0;  // doubles
     d += Nl * C * N * 2;                 // psi
-    d += Nl * C * nk * 2;                // Y
-    d += Nl * N * nk + Nl * nk;          // Q, lam
-    d += (size_t)p->n_m * Nthp * Nlp;    // B
+    const size_t es = p->precision ? 4 : 8;
+    size_t tbytes = (Nl * C * nk * 2 + Nl * N * nk + (size_t)p->n_m * Nthp * Nlp) * es;  // Y, Q, B
+    d += Nl * nk;                        // lam
     d += Nth * nh + 2 * Nth;             // PL, chi, omega
     d += 5 * N + p->n_orb;               // r, wr, v_gs, mask, rho_frozen, occ
     d += C * N * Nth;                    // dens
@@ -888,7 +945,7 @@ static size_t footprint_bytes(const ggb_problem *p, int Nlp, int Nthp, int ntile
     d += 2 * nI * N * Nth;               // phase (double2)
     d += 2 * nI * nh * N;                // q, V
     d += C * (size_t)ntiles * 3;         // obs_part
-    return d * sizeof(double) + (Nl + p->n_orb) * sizeof(int32_t);
+    return d * sizeof(double) + tbytes + (Nl + p->n_orb) * sizeof(int32_t);
 }
 
 extern "C" int ggb_footprint(const ggb_problem *p, uint64_t *bytes) {
@@ -912,7 +969,8 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     if (p->n_r < 4 || p->n_r > 4096 || p->n_l < 2 || p->n_l > 128 || p->n_theta < p->n_l ||
         p->n_theta > 256 || p->n_har < 1 || p->n_har > 256 || p->n_orb < 1 || p->n_inst < 1 ||
         p->n_m < 1 || p->n_m > p->n_l || p->nk_max < 16 || p->nk_max % 16 != 0 ||
-        p->nk_max > ((p->n_r + 15) / 16) * 16 || !(p->dt > 0.0) || p->precision != 0 ||
+        p->nk_max > ((p->n_r + 15) / 16) * 16 || !(p->dt > 0.0) ||
+        (p->precision != 0 && p->precision != 1) ||
         (long)p->n_orb * p->n_inst > 65535) {
         g_create_err = "ggb_create: problem sizes out of range";
         return GGB_E_ARG;
@@ -951,11 +1009,15 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     const size_t N = p->n_r, Nl = p->n_l, C = ctx->C, Nth = p->n_theta, nI = p->n_inst;
     bool ok = true;
     ok &= dalloc(&ctx->psi, Nl * C * N * 2) == hipSuccess;
-    ok &= dalloc(&ctx->Y, Nl * C * (size_t)p->nk_max * 2) == hipSuccess;
-    ok &= dalloc(&ctx->Q, Nl * N * (size_t)p->nk_max) == hipSuccess;
+    // Y, Q and B hold the compute type (8 or 4 bytes per element)
+    const size_t es = p->precision ? sizeof(float) : sizeof(double);
+    const size_t nY = Nl * C * (size_t)p->nk_max * 2, nQ = Nl * N * (size_t)p->nk_max;
+    const size_t nB = (size_t)p->n_m * ctx->Nthp * ctx->Nlp;
+    ok &= dalloc(&ctx->Y, (nY * es + 7) / 8) == hipSuccess;
+    ok &= dalloc(&ctx->Q, (nQ * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->lam, Nl * (size_t)p->nk_max) == hipSuccess;
     ok &= dalloc(&ctx->nkeep, Nl) == hipSuccess;
-    ok &= dalloc(&ctx->B, (size_t)p->n_m * ctx->Nthp * ctx->Nlp) == hipSuccess;
+    ok &= dalloc(&ctx->B, (nB * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->PL, Nth * (size_t)p->n_har) == hipSuccess;
     ok &= dalloc(&ctx->chi, Nth) == hipSuccess;
     ok &= dalloc(&ctx->omega, Nth) == hipSuccess;
@@ -974,10 +1036,9 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ok &= dalloc(&ctx->obs_part, C * (size_t)ctx->ntiles * 3) == hipSuccess;
     if (!ok) return bail("ggb_create: device allocation failed", GGB_E_NOMEM);
     // zero everything that kernels may read before it is written
-    if (hipMemset(ctx->B, 0, (size_t)p->n_m * ctx->Nthp * ctx->Nlp * sizeof(double)) != hipSuccess ||
-        hipMemset(ctx->Q, 0, Nl * N * (size_t)p->nk_max * sizeof(double)) != hipSuccess ||
+    if (hipMemset(ctx->B, 0, nB * es) != hipSuccess || hipMemset(ctx->Q, 0, nQ * es) != hipSuccess ||
         hipMemset(ctx->lam, 0, Nl * (size_t)p->nk_max * sizeof(double)) != hipSuccess ||
-        hipMemset(ctx->Y, 0, Nl * C * (size_t)p->nk_max * 2 * sizeof(double)) != hipSuccess ||
+        hipMemset(ctx->Y, 0, nY * es) != hipSuccess ||
         hipMemset(ctx->psi, 0, Nl * C * N * 2 * sizeof(double)) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess)
         return bail("ggb_create: hipMemset failed", GGB_E_HIP);
@@ -1007,14 +1068,29 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->wr, s->wr, N * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->v_gs, s->v_gs, N * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->mask, s->mask, N * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->Q, s->Q, Nl * N * p.nk_max * 8, hipMemcpyHostToDevice, st));
+    if (p.precision == 0) {
+        HIPCHK(ctx, hipMemcpyAsync(ctx->Q, s->Q, Nl * N * p.nk_max * 8, hipMemcpyHostToDevice, st));
+    } else {
+        std::vector<float> q32(Nl * N * p.nk_max);
+        for (size_t k = 0; k < q32.size(); ++k) q32[k] = (float)s->Q[k];
+        HIPCHK(ctx, hipMemcpy(ctx->Q, q32.data(), q32.size() * 4, hipMemcpyHostToDevice));
+    }
     HIPCHK(ctx, hipMemcpyAsync(ctx->lam, s->lam, Nl * p.nk_max * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->nkeep, s->nkeep, Nl * 4, hipMemcpyHostToDevice, st));
     // B is stored zero padded to [n_m][Nthp][Nlp]
-    for (int mm = 0; mm < p.n_m; ++mm)
-        HIPCHK(ctx, hipMemcpy2DAsync(ctx->B + (size_t)mm * ctx->Nthp * ctx->Nlp, ctx->Nlp * 8,
-                                     s->B + (size_t)mm * Nth * Nl, Nl * 8, Nl * 8, Nth,
-                                     hipMemcpyHostToDevice, st));
+    if (p.precision == 0) {
+        for (int mm = 0; mm < p.n_m; ++mm)
+            HIPCHK(ctx, hipMemcpy2DAsync(ctx->B + (size_t)mm * ctx->Nthp * ctx->Nlp, ctx->Nlp * 8,
+                                         s->B + (size_t)mm * Nth * Nl, Nl * 8, Nl * 8, Nth,
+                                         hipMemcpyHostToDevice, st));
+    } else {
+        std::vector<float> b32((size_t)p.n_m * ctx->Nthp * ctx->Nlp, 0.f);
+        for (int mm = 0; mm < p.n_m; ++mm)
+            for (size_t j = 0; j < Nth; ++j)
+                for (size_t l = 0; l < Nl; ++l)
+                    b32[((size_t)mm * ctx->Nthp + j) * ctx->Nlp + l] = (float)s->B[((size_t)mm * Nth + j) * Nl + l];
+        HIPCHK(ctx, hipMemcpy(ctx->B, b32.data(), b32.size() * 4, hipMemcpyHostToDevice));
+    }
     HIPCHK(ctx, hipMemcpyAsync(ctx->PL, s->PL, Nth * p.n_har * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->chi, s->chi, Nth * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->omega, s->omega, Nth * 8, hipMemcpyHostToDevice, st));
@@ -1058,7 +1134,18 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
     return g;
 }
 
-static size_t ang_lds(ggb_ctx *ctx) { return (size_t)(ctx->Nlp + ctx->Nthp) * 16 * sizeof(double); }
+static size_t ang_lds(ggb_ctx *ctx) {
+    return (size_t)(ctx->Nlp + ctx->Nthp) * 16 * (ctx->p.precision ? sizeof(float) : sizeof(double));
+}
+
+template <int MODE>
+static void launch_angular(ggb_ctx *ctx, int step) {
+    const dim3 ga(ctx->ntiles, ctx->C);
+    if (ctx->p.precision)
+        k_angular<MODE, float><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+    else
+        k_angular<MODE, double><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+}
 
 // potential rebuild from the |Psi|^2 tiles already in ctx->dens
 static int launch_potential(ggb_ctx *ctx) {
@@ -1066,9 +1153,14 @@ static int launch_potential(ggb_ctx *ctx) {
     hipStream_t st = ctx->stream;
     const dim3 gt(ctx->ntiles, p.n_inst);
     if (p.interacting) {
-        k_density<<<gt, WG, (size_t)RT * p.n_theta * 8, st>>>(
-            ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PL, ctx->vxc,
-            ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
+        if (p.precision)
+            k_density<float><<<gt, WG, (size_t)RT * p.n_theta * 8, st>>>(
+                ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PL,
+                ctx->vxc, ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
+        else
+            k_density<double><<<gt, WG, (size_t)RT * p.n_theta * 8, st>>>(
+                ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PL,
+                ctx->vxc, ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
         k_hartree<<<dim3(p.n_har, p.n_inst), WG, 0, st>>>(ctx->q, ctx->r, ctx->V, p.n_r, p.n_har);
     }
     k_dv<<<gt, WG, (size_t)p.n_har * RT * 8, st>>>(ctx->V, ctx->vxc, ctx->PL, ctx->v_gs, ctx->mask,
@@ -1079,29 +1171,40 @@ static int launch_potential(ggb_ctx *ctx) {
 }
 
 static int launch_first_half(ggb_ctx *ctx, int step) {
-    const dim3 ga(ctx->ntiles, ctx->C);
-    k_angular<0><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+    launch_angular<0>(ctx, step);
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
 }
 
-static int launch_kinetic_pass(ggb_ctx *ctx, int pass) {
+template <typename T>
+static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
     const ggb_problem &p = ctx->p;
-    const int ncol = (2 * ctx->C + 15) / 16;
+    const T *Q = reinterpret_cast<const T *>(ctx->Q);
+    T *Y = reinterpret_cast<T *>(ctx->Y);
     if (ctx->C > KIN_NARROW_MAX_C) {
         const int nct = (2 * ctx->C + TB - 1) / TB;
         if (pass == 0)
-            k_kin_fwd_tiled<<<dim3((p.nk_max + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
-                ctx->Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->Y, p.n_r, p.nk_max, ctx->C, p.dt);
+            k_kin_fwd_tiled<T><<<dim3((p.nk_max + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
+                Q, ctx->lam, ctx->nkeep, ctx->psi, Y, p.n_r, p.nk_max, ctx->C, p.dt);
         else
-            k_kin_bwd_tiled<<<dim3((p.n_r + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
-                ctx->Q, ctx->nkeep, ctx->Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
-    } else if (pass == 0)
-        k_kin_fwd<<<dim3((p.nk_max + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
-            ctx->Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->Y, p.n_r, p.nk_max, ctx->C, p.dt);
+            k_kin_bwd_tiled<T><<<dim3((p.n_r + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
+                Q, ctx->nkeep, Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
+        return;
+    }
+    const int ncol = (2 * ctx->C + 15) / 16;
+    if (pass == 0)
+        k_kin_fwd<T><<<dim3((p.nk_max + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
+            Q, ctx->lam, ctx->nkeep, ctx->psi, Y, p.n_r, p.nk_max, ctx->C, p.dt);
     else
-        k_kin_bwd<<<dim3((p.n_r + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
-            ctx->Q, ctx->nkeep, ctx->Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
+        k_kin_bwd<T><<<dim3((p.n_r + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
+            Q, ctx->nkeep, Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
+}
+
+static int launch_kinetic_pass(ggb_ctx *ctx, int pass) {
+    if (ctx->p.precision)
+        launch_kinetic_t<float>(ctx, pass);
+    else
+        launch_kinetic_t<double>(ctx, pass);
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
 }
@@ -1112,8 +1215,7 @@ static int launch_kinetic(ggb_ctx *ctx) {
 }
 
 static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst) {
-    const dim3 ga(ctx->ntiles, ctx->C);
-    k_angular<1><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+    launch_angular<1>(ctx, step);
     if (obs_dst)
         k_obs_reduce<<<(ctx->C * 3 + 255) / 256, 256, 0, ctx->stream>>>(ctx->obs_part, obs_dst,
                                                                          ctx->C, ctx->ntiles);
@@ -1135,8 +1237,7 @@ extern "C" int ggb_upload_orbitals(ggb_ctx *ctx, const double *psi) {
         HIPCHK(ctx, hipMemsetAsync(ctx->E, 0, (size_t)p.n_inst * 8, ctx->stream));
         ctx->n_field = 1;
     }
-    const dim3 ga(ctx->ntiles, ctx->C);
-    k_angular<2><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, 0));
+    launch_angular<2>(ctx, 0);
     HIPCHK(ctx, hipGetLastError());
     int rc = launch_potential(ctx);
     if (rc) return rc;
@@ -1163,8 +1264,7 @@ extern "C" int ggb_rebuild_potential(ggb_ctx *ctx) {
     if (!ctx) return GGB_E_ARG;
     if (!ctx->have_orbitals) return fail(ctx, GGB_E_STATE, "ggb_rebuild_potential: no orbitals");
     HIPCHK(ctx, hipSetDevice(ctx->p.device));
-    const dim3 ga(ctx->ntiles, ctx->C);
-    k_angular<2><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, 0));
+    launch_angular<2>(ctx, 0);
     HIPCHK(ctx, hipGetLastError());
     int rc = launch_potential(ctx);
     if (rc) return rc;
@@ -1334,15 +1434,13 @@ extern "C" int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step) {
         case 0: rc = launch_first_half(ctx, step); break;
         case 1: rc = launch_kinetic(ctx); break;
         case 2: {
-            const dim3 ga(ctx->ntiles, ctx->C);
-            k_angular<1><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+            launch_angular<1>(ctx, step);
             HIPCHK(ctx, hipGetLastError());
             rc = GGB_OK;
             break;
         }
         case 3: {
-            const dim3 ga(ctx->ntiles, ctx->C);
-            k_angular<2><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+            launch_angular<2>(ctx, step);
             HIPCHK(ctx, hipGetLastError());
             rc = launch_potential(ctx);
             break;
@@ -1358,10 +1456,10 @@ extern "C" int ggb_selftest_mfma(int32_t device, double *out) {
     if (!out) return GGB_E_ARG;
     if (hipSetDevice(device) != hipSuccess) return GGB_E_HIP;
     double *d = nullptr;
-    if (hipMalloc((void **)&d, 256 * sizeof(double)) != hipSuccess) return GGB_E_NOMEM;
+    if (hipMalloc((void **)&d, 512 * sizeof(double)) != hipSuccess) return GGB_E_NOMEM;
     k_selftest_mfma<<<1, 64>>>(d);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpy(out, d, 256 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out, d, 512 * sizeof(double), hipMemcpyDeviceToHost);
     hipFree(d);
     return e == hipSuccess ? GGB_OK : GGB_E_HIP;
 }

@@ -11,7 +11,7 @@ import numpy as np
 from . import _lib
 
 
-def problem_for(system, n_inst=1, device=0):
+def problem_for(system, n_inst=1, device=0, precision=0):
     """ggb_problem of a context holding n_inst instances of `system`."""
     p = _lib.Problem()
     p.abi_version = _lib.GGB_ABI_VERSION
@@ -19,15 +19,15 @@ def problem_for(system, n_inst=1, device=0):
     p.n_har, p.n_orb, p.n_inst = system.n_har, system.n_orb, int(n_inst)
     p.n_m, p.nk_max = system.angular.n_m, system.nk_max
     p.interacting = 1 if system.interacting else 0
-    p.precision, p.device, p.dt = 0, int(device), system.dt
+    p.precision, p.device, p.dt = int(precision), int(device), system.dt
     return p
 
 
-def footprint(system, n_inst=1, field_steps=0, obs_steps=0):
+def footprint(system, n_inst=1, field_steps=0, obs_steps=0, precision=0):
     """Device bytes of a context (ggb_footprint) plus its field and observable buffers."""
     lib = _lib.load()
     b = ctypes.c_uint64()
-    p = problem_for(system, n_inst)
+    p = problem_for(system, n_inst, 0, precision)
     _lib.check(lib.ggb_footprint(ctypes.byref(p), ctypes.byref(b)), None, "ggb_footprint")
     return int(b.value) + 8 * int(n_inst) * int(field_steps) + \
         24 * system.n_orb * int(n_inst) * int(obs_steps)
@@ -40,12 +40,13 @@ class Propagator:
     has its own orbitals, field and potentials.
     """
 
-    def __init__(self, system, n_inst=1, device=0):
+    def __init__(self, system, n_inst=1, device=0, precision=0):
         self.lib = _lib.load()
+        self.precision = int(precision)
         self.sys = system
         self.n_inst = int(n_inst)
         self.C = system.n_orb * self.n_inst
-        p = problem_for(system, self.n_inst, device)
+        p = problem_for(system, self.n_inst, device, self.precision)
         self._problem = p
         ctx = ctypes.c_void_p()
         _lib.check(self.lib.ggb_create(ctypes.byref(p), ctypes.byref(ctx)), None, "ggb_create")
@@ -170,7 +171,8 @@ class Propagator:
 
 
 def selftest_mfma(device=0):
+    """[2, 16, 16]: the FP64 and FP32 16x16x4 matrix-core products."""
     lib = _lib.load()
-    out = np.empty((16, 16))
+    out = np.empty((2, 16, 16))
     _lib.check(lib.ggb_selftest_mfma(device, _lib.dptr(out)), None, "ggb_selftest_mfma")
     return out

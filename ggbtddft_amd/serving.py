@@ -139,12 +139,15 @@ class Server:
     """Continuous batching over lanes; see the module docstring."""
 
     def __init__(self, memory_budget=64 << 30, slots=8, chunk=256, ring=1 << 16,
-                 device=0, factory=None, system_factory=None, spectrum_beta=24.0):
+                 device=0, factory=None, system_factory=None, spectrum_beta=24.0,
+                 precision=0):
+        import functools
         from . import engine
         self.memory_budget = int(memory_budget)
         self.slots, self.chunk, self.ring, self.device = slots, chunk, ring, device
-        self.factory = factory or engine.Propagator
-        self.footprint = (lambda sysm: engine.footprint(sysm, slots, ring, chunk)) \
+        self.precision = int(precision)
+        self.factory = factory or functools.partial(engine.Propagator, precision=self.precision)
+        self.footprint = (lambda sysm: engine.footprint(sysm, slots, ring, chunk, self.precision)) \
             if factory is None else (lambda sysm: 0)
         self.system_factory = system_factory or (lambda k: AtomSystem(k[0], k[1], k[2], k[3], k[4]))
         self.systems = {}

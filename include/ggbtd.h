@@ -55,7 +55,10 @@ typedef struct ggb_problem {
     int32_t n_m;          /* distinct |m| values; orbital m in [0, n_m) */
     int32_t nk_max;       /* row length of the eigenvector table, multiple of 16 */
     int32_t interacting;  /* 1: Hartree + LDA update each step; 0: dV = 0 */
-    int32_t precision;    /* 0: FP64 (only value accepted by this version) */
+    int32_t precision;    /* 0: FP64 everywhere; 1: kinetic tables, transforms and
+                             v_xc in FP32 on the FP32 matrix cores with FP32
+                             accumulation; state, Hartree scans and observable
+                             sums stay FP64 */
     int32_t device;       /* HIP device ordinal */
     double dt;            /* time step (a.u.) */
 } ggb_problem;
@@ -161,8 +164,9 @@ int ggb_set_timing(ggb_ctx *ctx, int32_t mode);
 /* Accumulated milliseconds and launch counts per phase [GGB_NPHASE]. */
 int ggb_get_timing(const ggb_ctx *ctx, double *ms_total, int64_t *count);
 
-/* 16x16x4 FP64 matrix-core product of fixed inputs A[i][k] = i + 0.25 k,
- * B[k][j] = 0.5 k - j, written to out[16][16] (lane-layout self test). */
+/* 16x16x4 matrix-core products of fixed inputs A[i][k] = i + 0.25 k,
+ * B[k][j] = 0.5 k - j, in FP64 (out[0][16][16]) and FP32 (out[1][16][16])
+ * (lane-layout self test of the instructions the kernels use). */
 int ggb_selftest_mfma(int32_t device, double *out);
 
 /* Copy `bytes` (multiple of 16) device-to-device with a plain streaming

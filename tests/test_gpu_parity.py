@@ -51,7 +51,8 @@ def test_mfma_lane_layout():
     k = np.arange(4)[None, :]
     A = i + 0.25 * k
     B = 0.5 * np.arange(4)[:, None] - np.arange(16)[None, :]
-    np.testing.assert_allclose(out, A @ B, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(out[0], A @ B, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(out[1], A @ B, rtol=1e-6, atol=1e-5)
 
 
 @pytest.mark.parametrize("atom,N,R,lmax", [("He", 32, 30.0, 4), ("Ar", 48, 40.0, 6)])
@@ -137,3 +138,23 @@ def test_batched_instances_match_alone():
             assert d <= 1e-14 * max(1.0, np.abs(alone[key]).max()), (k, key, d)
         cols = slice(k * S.n_orb, (k + 1) * S.n_orb)
         assert np.abs(psi_mix[:, cols, :] - psi_alone).max() < 1e-14
+
+
+@pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30)])
+def test_fp32_short_run(atom, N, R, lmax, nsteps):
+    """Mixed precision (FP32 tables/transforms/v_xc, FP64 state and sums)
+    against the FP64 oracle: FP32-level agreement."""
+    _say("case: fp32 short run", atom, N, lmax, nsteps)
+    dt = 0.05
+    S, O = _pair(atom, N, R, lmax, dt)
+    w = OF.omega_from_wavelength(800.0)
+    Em = OF.midpoint_series(OF.e0_from_intensity(5e14), w, 1, dt)[:nsteps] + 0.05
+    ref = O.run(Em)
+    with _engine().Propagator(S, precision=1) as prop:
+        prop.set_field(Em)
+        out = prop.run(0, nsteps)
+    for key in ("dip", "acc"):
+        rel = np.linalg.norm(out[key][:, 0, :] - ref[key]) / np.linalg.norm(ref[key])
+        _say("fp32", key, "rel", rel)
+        assert rel < 1e-4, (key, rel)
+    assert np.abs(out["norm"][:, 0, :] - ref["norm"]).max() < 1e-5
