@@ -61,6 +61,65 @@ def cpu_baseline(system_args, E, nsteps=40, warm=3):
                        "steps, %.1f s (setup %.1f s excluded)" % (nsteps, warm, el, setup))
 
 
+def batched_run(sysm, pulse, args, fp64_value):
+    """n Ar instances (intensities log-spaced 1e14..3e14) in one context, FP32."""
+    from ggbtddft_amd.engine import Propagator
+    from ggbtddft_amd.pulse import Pulse
+    n = args.batched
+    Is = np.geomspace(1e14, 3e14, n)
+    nst = args.warmup + args.batched_steps
+    E = np.stack([Pulse(I, WAVELENGTH, CYCLES).midpoints(DT)[:nst] for I in Is])
+    prop = Propagator(sysm, n_inst=n, precision=1)
+    prop.set_field(E)
+    prop.run(0, args.warmup)
+    t0 = time.perf_counter()
+    obs = prop.run(args.warmup, args.batched_steps)
+    el = time.perf_counter() - t0
+    prop.close()
+    assert np.all(np.isfinite(obs["acc"]))
+    val = n * sysm.n_orb * args.batched_steps / el
+    log("batched fp32: %d instances, %d steps in %.2f s" % (n, args.batched_steps, el))
+    return {"value": val, "unit": "orbital-timesteps/s", "instances": n,
+            "steps": args.batched_steps, "ms_per_step": 1e3 * el / args.batched_steps,
+            "precision": "fp32 tables/transforms/v_xc, fp64 state and sums",
+            "ratio_vs_fp64_plain": val / fp64_value}
+
+
+def precision_report(sysm, pulse):
+    """Full-length runs in FP64 and FP32; errors of the FP32 spectrum."""
+    from ggbtddft_amd import spectrum as sp
+    from ggbtddft_amd.engine import Propagator
+    n = pulse.n_steps(DT)
+    E = pulse.midpoints(DT)
+    E_end = pulse((np.arange(n) + 1.0) * DT)
+    out = {}
+    for prec in (0, 1):
+        t0 = time.perf_counter()
+        with Propagator(sysm, precision=prec) as prop:
+            prop.set_field(E)
+            o = prop.run(0, n)
+        out[prec] = (sysm.total_acceleration(o["acc"][:, 0, :], o["norm"][:, 0, :], E_end),
+                     time.perf_counter() - t0)
+        log("precision %d full run: %d steps %.1f s" % (prec, n, out[prec][1]))
+    a64, a32 = out[0][0], out[1][0]
+    rel = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+    w, S64 = sp.acceleration_spectrum(a64, DT)
+    _, S32 = sp.acceleration_spectrum(a32, DT)
+    ip = -float(sysm.gs.eps.max())
+    cut = sp.cutoff_energy(pulse.E0, pulse.omega0, ip) / pulse.omega0
+    orders = np.arange(3, int(cut) + 8, 2)
+    I64 = sp.harmonic_intensities(w, S64, pulse.omega0, orders)
+    I32 = sp.harmonic_intensities(w, S32, pulse.omega0, orders)
+    err = np.log10(I32 / I64)
+    plateau = orders < 0.8 * cut
+    return {"steps": n, "acc_rel_l2": rel, "cutoff_order": cut,
+            "orders": orders.tolist(), "log10_intensity_error": err.tolist(),
+            "max_abs_log10_err_plateau": float(np.abs(err[plateau]).max()),
+            "max_abs_log10_err_cutoff": float(np.abs(err[~plateau]).max()),
+            "one_percent_in_log10": float(np.log10(1.01)),
+            "wall_s_fp64": out[0][1], "wall_s_fp32": out[1][1]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +127,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--batched", type=int, default=32,
+                    help="Ar instances for the batched FP32 leg (0 = skip)")
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--precision-report", action="store_true",
+                    help="full-length FP64 and FP32 runs; harmonic-intensity errors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (profiler byte-counter "
                          "calibration) and print its bandwidth")
@@ -152,6 +216,10 @@ def main():
                      "bytes_per_launch": bytes_qt, "avg_launch_ms": avg_qt,
                      "avg_launch_ms_q_pass": avg_q},
     }
+    if args.batched > 0:
+        line["batched_fp32"] = batched_run(sysm, pulse, args, value)
+    if args.precision_report:
+        line["precision_report"] = precision_report(sysm, pulse)
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline((ATOM, N, R_MAX, LMAX, DT), E,
