@@ -397,6 +397,7 @@ struct AngArgs {
     const double *E;       // [n_inst][n_field]
     double *dens;          // [C][N][Nth] |Psi|^2
     double *obs_part;      // [C][ntiles][3]
+    const int *cur;        // graph replay: field step = (cur[0] + step) mod n_field
     int N, Nl, Nlp, Nth, Nthp, C, n_orb, n_field, step;
     double dt;
 };
@@ -519,7 +520,8 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
     }
     if (MODE != 2 && t < RT) {
         const int i = i0 + t;
-        const double Ev = g.E[(size_t)inst * g.n_field + g.step];
+        const int fs = g.cur ? (int)(((long)g.cur[0] + g.step) % g.n_field) : g.step;
+        const double Ev = g.E[(size_t)inst * g.n_field + fs];
         a_i[t] = (i < g.N) ? 0.5 * g.dt * Ev * g.r[i] : 0.0;
     }
     __syncthreads();
@@ -629,18 +631,22 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
 
 // per-column sums of the observable partials -> obs[step_rel][c][3]
 __global__ void k_obs_reduce(const double *__restrict__ obs_part, double *__restrict__ obs,
-                             int C, int ntiles) {
+                             int C, int ntiles, const int *cur, const int *origin, int local) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= C * 3) return;
     const int c = idx / 3, k = idx % 3;
     double s = 0.0;
     for (int tl = 0; tl < ntiles; ++tl) s += obs_part[((size_t)c * ntiles + tl) * 3 + k];
-    obs[idx] = s;
+    // graph replay: the destination row is the step's index within the run call
+    const size_t row = cur ? (size_t)(cur[0] - origin[0] + local) : 0;
+    obs[row * C * 3 + idx] = s;
 }
 
-// ---------------------------------------------------------------------------
-// Density, LDA v_xc and multipole moments for one instance and RT radial nodes.
-// ---------------------------------------------------------------------------
+// advances the replayed graph's step counter
+__global__ void k_bump(int *cur, int by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) cur[0] += by;
+}
+
 template <typename T> struct lda_t;
 template <> struct lda_t<double> {
     static __device__ double cbrt_(double x) { return cbrt(x); }
@@ -879,6 +885,10 @@ struct ggb_ctx {
     double *dens, *vxc, *q, *V, *dv, *obs_part, *obs, *E;
     double2 *phase;
     size_t obs_cap;
+    // optional hipGraph of graph_steps steps (ggb_set_graph)
+    int graph_steps;
+    hipGraphExec_t graph_exec;
+    int *dcur;  // [0] = step counter of the replay, [1] = first step of the run call
     // optional HIP-event timing of the step phases (ggb_set_timing)
     int timing;                     // 0 off, 1 kinetic only, 2 every phase
     std::vector<hipEvent_t> events; // GGB_NPHASE + 1 per step of a run call
@@ -900,6 +910,15 @@ static std::string g_create_err;
 static int fail(ggb_ctx *ctx, int code, const std::string &msg) {
     ctx->err = msg;
     return code;
+}
+
+// a captured graph holds raw buffer pointers: drop it whenever one may move
+static void drop_graph(ggb_ctx *ctx) {
+    if (ctx->graph_exec) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+    }
 }
 
 template <typename T>
@@ -926,6 +945,8 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+    if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->dcur) hipFree(ctx->dcur);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -993,6 +1014,9 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->phase = nullptr;
     ctx->stream = nullptr;
     ctx->timing = 0;
+    ctx->graph_steps = 0;
+    ctx->graph_exec = nullptr;
+    ctx->dcur = nullptr;
     for (int k = 0; k < GGB_NPHASE; ++k) {
         ctx->phase_ms[k] = 0.0;
         ctx->phase_n[k] = 0;
@@ -1062,6 +1086,7 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
         if (!(s->r[i] > 0.0) || !(s->wr[i] > 0.0))
             return fail(ctx, GGB_E_ARG, "ggb_upload_static: r and wr must be positive");
     HIPCHK(ctx, hipSetDevice(p.device));
+    drop_graph(ctx);
     const size_t N = p.n_r, Nl = p.n_l, Nth = p.n_theta;
     hipStream_t st = ctx->stream;
     HIPCHK(ctx, hipMemcpyAsync(ctx->r, s->r, N * 8, hipMemcpyHostToDevice, st));
@@ -1131,6 +1156,7 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
     g.n_field = ctx->n_field;
     g.step = step;
     g.dt = ctx->p.dt;
+    g.cur = nullptr;
     return g;
 }
 
@@ -1139,12 +1165,14 @@ static size_t ang_lds(ggb_ctx *ctx) {
 }
 
 template <int MODE>
-static void launch_angular(ggb_ctx *ctx, int step) {
+static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
     const dim3 ga(ctx->ntiles, ctx->C);
+    AngArgs g = ang_args(ctx, step);
+    g.cur = cur;
     if (ctx->p.precision)
-        k_angular<MODE, float><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+        k_angular<MODE, float><<<ga, WG, ang_lds(ctx), ctx->stream>>>(g);
     else
-        k_angular<MODE, double><<<ga, WG, ang_lds(ctx), ctx->stream>>>(ang_args(ctx, step));
+        k_angular<MODE, double><<<ga, WG, ang_lds(ctx), ctx->stream>>>(g);
 }
 
 // potential rebuild from the |Psi|^2 tiles already in ctx->dens
@@ -1170,8 +1198,8 @@ static int launch_potential(ggb_ctx *ctx) {
     return GGB_OK;
 }
 
-static int launch_first_half(ggb_ctx *ctx, int step) {
-    launch_angular<0>(ctx, step);
+static int launch_first_half(ggb_ctx *ctx, int step, const int *cur = nullptr) {
+    launch_angular<0>(ctx, step, cur);
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
 }
@@ -1214,11 +1242,11 @@ static int launch_kinetic(ggb_ctx *ctx) {
     return rc ? rc : launch_kinetic_pass(ctx, 1);
 }
 
-static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst) {
-    launch_angular<1>(ctx, step);
+static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int *cur = nullptr) {
+    launch_angular<1>(ctx, step, cur);
     if (obs_dst)
-        k_obs_reduce<<<(ctx->C * 3 + 255) / 256, 256, 0, ctx->stream>>>(ctx->obs_part, obs_dst,
-                                                                         ctx->C, ctx->ntiles);
+        k_obs_reduce<<<(ctx->C * 3 + 255) / 256, 256, 0, ctx->stream>>>(
+            ctx->obs_part, obs_dst, ctx->C, ctx->ntiles, cur, cur ? cur + 1 : nullptr, step);
     HIPCHK(ctx, hipGetLastError());
     return launch_potential(ctx);
 }
@@ -1278,6 +1306,7 @@ extern "C" int ggb_set_field(ggb_ctx *ctx, const double *E, int32_t n_steps) {
     const ggb_problem &p = ctx->p;
     HIPCHK(ctx, hipSetDevice(p.device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
     if (ctx->E) {
         hipFree(ctx->E);
         ctx->E = nullptr;
@@ -1297,6 +1326,7 @@ extern "C" int ggb_field_ring(ggb_ctx *ctx, int32_t capacity) {
     const ggb_problem &p = ctx->p;
     HIPCHK(ctx, hipSetDevice(p.device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
     if (ctx->E) {
         hipFree(ctx->E);
         ctx->E = nullptr;
@@ -1328,6 +1358,49 @@ extern "C" int ggb_set_field_slice(ggb_ctx *ctx, int32_t inst, int32_t start, in
     return GGB_OK;
 }
 
+// Capture graph_steps steps once (field index and observable row taken from
+// the device counter dcur), then replay it; returns the number of steps done.
+static int run_graph(ggb_ctx *ctx, int step0, int nsteps, int *done) {
+    const int G = ctx->graph_steps;
+    if (!ctx->dcur) HIPCHK(ctx, hipMalloc((void **)&ctx->dcur, 2 * sizeof(int)));
+    if (!ctx->graph_exec) {
+        hipGraph_t graph;
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = GGB_OK;
+        for (int s = 0; s < G && !rc; ++s) {
+            rc = launch_first_half(ctx, s, ctx->dcur);
+            if (!rc) rc = launch_kinetic(ctx);
+            if (!rc) rc = launch_second_half(ctx, s, ctx->obs, ctx->dcur);
+        }
+        if (!rc) k_bump<<<1, 64, 0, ctx->stream>>>(ctx->dcur, G);
+        hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(ctx, GGB_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            ctx->graph_exec = nullptr;
+            return fail(ctx, GGB_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+    }
+    int h[2] = {step0, step0};
+    HIPCHK(ctx, hipMemcpyAsync(ctx->dcur, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    const int reps = nsteps / G;
+    for (int k = 0; k < reps; ++k) HIPCHK(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    *done = reps * G;
+    return GGB_OK;
+}
+
+extern "C" int ggb_set_graph(ggb_ctx *ctx, int32_t steps) {
+    if (!ctx) return GGB_E_ARG;
+    if (steps < 0 || steps > 4096) return fail(ctx, GGB_E_ARG, "ggb_set_graph: steps in [0, 4096]");
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->graph_steps = steps;
+    return GGB_OK;
+}
+
 extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip, double *acc,
                        double *norm) {
     if (!ctx) return GGB_E_ARG;
@@ -1338,6 +1411,7 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
     HIPCHK(ctx, hipSetDevice(ctx->p.device));
     const size_t need = (size_t)nsteps * ctx->C * 3;
     if (need > ctx->obs_cap) {
+        drop_graph(ctx);
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->obs) hipFree(ctx->obs);
         ctx->obs = nullptr;
@@ -1359,7 +1433,12 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         if (ctx->timing == 1 && k != 1 && k != 2 && k != 3) return hipSuccess;
         return hipEventRecord(ctx->events[(size_t)s * ne + k], ctx->stream);
     };
-    for (int s = 0; s < nsteps; ++s) {
+    int s_begin = 0;
+    if (ctx->graph_steps > 0 && !ctx->timing && nsteps >= ctx->graph_steps) {
+        int rc = run_graph(ctx, step0, nsteps, &s_begin);
+        if (rc) return rc;
+    }
+    for (int s = s_begin; s < nsteps; ++s) {
         const int step = (int)(((long)step0 + s) % ctx->n_field);
         HIPCHK(ctx, mark(s, 0));
         int rc = launch_first_half(ctx, step);

@@ -136,6 +136,10 @@ class Propagator:
         sh = (nsteps, self.n_inst, self.sys.n_orb)
         return dict(dip=dip.reshape(sh), acc=acc.reshape(sh), norm=nrm.reshape(sh))
 
+    def set_graph(self, steps):
+        """Replay the step loop from a hipGraph of `steps` steps (0 = off)."""
+        self._check(self.lib.ggb_set_graph(self.ctx, int(steps)), "ggb_set_graph")
+
     def set_timing(self, mode):
         """0 off, 1 kinetic passes, 2 every phase (HIP events on the context stream)."""
         self._check(self.lib.ggb_set_timing(self.ctx, int(mode)), "ggb_set_timing")

@@ -150,6 +150,12 @@ int ggb_download_potential(ggb_ctx *ctx, double *dv);
  * current state.  `step` selects the field sample. */
 int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step);
 
+/* Replay the step loop of ggb_run from a hipGraph of `steps` steps (0 =
+ * plain stream launches, the default).  The graph is captured on the next
+ * ggb_run and re-captured after any call that may move a buffer; runs with
+ * timing enabled use plain launches.  No counterpart in the reference. */
+int ggb_set_graph(ggb_ctx *ctx, int32_t steps);
+
 /* Step phases timed by ggb_set_timing: 0 = half potential + laser,
  * 1 = kinetic pass Q^T, 2 = kinetic pass Q, 3 = laser + half potential +
  * observables, 4 = potential rebuild. */
