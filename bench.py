@@ -176,6 +176,11 @@ def main():
     log("timed %d steps in %.2f s" % (args.steps, el))
     tim = prop.timing()
     psi = prop.download_orbitals()
+    # per-phase breakdown in a separate pass (events between every phase)
+    prop.set_timing(2)
+    nb = min(200, nfull - need)
+    prop.run(need, nb)
+    phases = {k: v[0] / max(v[1], 1) for k, v in prop.timing().items()}
     prop.close()
     assert np.all(np.isfinite(obs["acc"])) and np.all(np.isfinite(psi))
 
@@ -210,6 +215,7 @@ def main():
                    "n_theta": sysm.n_theta, "orbitals": norb, "dt": DT,
                    "pulse": "800 nm sin^2 8 cycles 2e14 W/cm^2", "full_run_steps": nfull,
                    "precision": "fp64", "parallelism": "single GPU"},
+        "phase_ms_per_step": phases,
         "roofline": {"kernel": "k_kin_fwd (kinetic propagator, Q^T pass)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

