@@ -359,7 +359,9 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
         if (more) load(k0 + TK);
 #pragma unroll
         for (int ks = 0; ks < TK / 4; ++ks) {
-            const int kk = ks * 4 + kr;
+            // lane kr of MFMA ks takes a = k0 + 4 kr + ks: the K order of k_kin_bwd,
+            // so an instance's result does not depend on which kernel serves it
+            const int kk = 4 * kr + ks;
             const T a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
             const T b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
             acc[0][0] = mfma_t<T>::op(a0v, b0v, acc[0][0]);
@@ -1253,12 +1255,17 @@ static int launch_kinetic(ggb_ctx *ctx) {
     return rc ? rc : launch_kinetic_pass(ctx, 1);
 }
 
-static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int *cur = nullptr) {
+static hipError_t no_mark() { return hipSuccess; }
+
+template <typename Mark = hipError_t (*)()>
+static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int *cur = nullptr,
+                              Mark between = no_mark) {
     launch_angular<1>(ctx, step, cur);
     if (obs_dst)
         k_obs_reduce<<<(ctx->C * 3 + 255) / 256, 256, 0, ctx->stream>>>(
             ctx->obs_part, obs_dst, ctx->C, ctx->ntiles, cur, cur ? cur + 1 : nullptr, step);
     HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, between());  // timing event between the angular half and the rebuild
     return launch_potential(ctx);
 }
 
@@ -1441,7 +1448,7 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
     }
     auto mark = [&](int s, int k) -> hipError_t {
         if (!ctx->timing) return hipSuccess;
-        if (ctx->timing == 1 && k != 1 && k != 2 && k != 3) return hipSuccess;
+        if (ctx->timing == 1 && k != 1 && k != 2 && k != 3) return hipSuccess;  // kinetic only
         return hipEventRecord(ctx->events[(size_t)s * ne + k], ctx->stream);
     };
     int s_begin = 0;
@@ -1458,8 +1465,9 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         HIPCHK(ctx, mark(s, 2));
         if (!rc) rc = launch_kinetic_pass(ctx, 1);
         HIPCHK(ctx, mark(s, 3));
-        if (!rc) rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3);
-        HIPCHK(ctx, mark(s, 4));
+        if (!rc) rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3, nullptr,
+                                         [&]() { return mark(s, 4); });
+        HIPCHK(ctx, mark(s, 5));
         if (rc) return rc;
     }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

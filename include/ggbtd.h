@@ -158,7 +158,7 @@ int ggb_set_graph(ggb_ctx *ctx, int32_t steps);
 
 /* Step phases timed by ggb_set_timing: 0 = half potential + laser,
  * 1 = kinetic pass Q^T, 2 = kinetic pass Q, 3 = laser + half potential +
- * observables, 4 = potential rebuild. */
+ * observables, 4 = potential rebuild (density, v_xc, Hartree, phases). */
 #define GGB_NPHASE 5
 
 /* Record HIP events on the context stream around the step phases of later

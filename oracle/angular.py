@@ -31,47 +31,72 @@ def c_lm(l, m):
     return np.where(l >= m, val, 0.0)
 
 
-def theta_grid(n_theta):
-    """Gauss-Legendre nodes (ascending) and weights on [-1, 1]."""
-    chi, omega = np.polynomial.legendre.leggauss(n_theta)
-    return chi, omega
+EXT = np.longdouble   # tables are built in extended precision, rounded once
+
+
+def theta_grid(n_theta, extended=False):
+    """Gauss-Legendre nodes (ascending) and weights on [-1, 1]: numpy's
+    leggauss refined by Newton steps on P_n in extended precision, weights
+    2/((1-x^2) P_n'^2).  Rounded to double unless extended=True."""
+    chi = np.polynomial.legendre.leggauss(n_theta)[0].astype(EXT)
+    for it in range(4):
+        pm, p = np.ones_like(chi), chi.copy()
+        for k in range(1, n_theta):
+            pm, p = p, ((2 * k + 1) * chi * p - k * pm) / (k + 1)
+        dp = n_theta * (chi * p - pm) / (chi * chi - 1)
+        if it < 3:
+            chi = chi - p / dp
+    omega = 2 / ((1 - chi * chi) * dp * dp)
+    if extended:
+        return chi, omega
+    return chi.astype(np.float64), omega.astype(np.float64)
 
 
 def assoc_legendre_normalised(lmax, m, x):
-    """Theta_lm(x) for l = 0..lmax as an array [len(x), lmax+1] (zero for l<m)."""
-    x = np.asarray(x, dtype=np.float64)
-    out = np.zeros((x.size, lmax + 1))
+    """Theta_lm(x) for l = 0..lmax as an array [len(x), lmax+1] (zero for l<m),
+    computed in the precision of x (pass extended-precision nodes to get
+    tables rounded once)."""
+    x = np.asarray(x)
+    if x.dtype != EXT:
+        x = x.astype(np.float64)
+    out = np.zeros((x.size, lmax + 1), dtype=x.dtype)
     if m > lmax:
         return out
     # Theta_mm = sqrt((2m+1)/2 * (2m)!/(2^m m!)^2 ... ) (1-x^2)^{m/2}, built
     # iteratively: Theta_00 = 1/sqrt(2), Theta_kk = sqrt((2k+1)/(2k)) s Theta_{k-1,k-1}
-    s = np.sqrt(np.maximum(1.0 - x * x, 0.0))
-    t = np.full_like(x, 1.0 / np.sqrt(2.0))
+    one = x.dtype.type(1)
+    s = np.sqrt(np.maximum(one - x * x, 0))
+    t = np.full_like(x, one / np.sqrt(x.dtype.type(2)))
     for k in range(1, m + 1):
-        t = np.sqrt((2.0 * k + 1.0) / (2.0 * k)) * s * t
+        t = np.sqrt(x.dtype.type(2 * k + 1) / x.dtype.type(2 * k)) * s * t
     out[:, m] = t
+    c = lambda l: np.sqrt(x.dtype.type((l + 1) ** 2 - m * m) / x.dtype.type((2 * l + 1) * (2 * l + 3)))
     if m + 1 <= lmax:
-        out[:, m + 1] = x * out[:, m] / c_lm(m, m)
+        out[:, m + 1] = x * out[:, m] / c(m)
     for l in range(m + 1, lmax):
-        out[:, l + 1] = (x * out[:, l] - c_lm(l - 1, m) * out[:, l - 1]) / c_lm(l, m)
+        out[:, l + 1] = (x * out[:, l] - c(l - 1) * out[:, l - 1]) / c(l)
     return out
 
 
 def transform_matrix(lmax, m, chi, omega):
-    """B[j, l] = sqrt(omega_j) Theta_lm(chi_j)."""
-    return np.sqrt(omega)[:, None] * assoc_legendre_normalised(lmax, m, chi)
+    """B[j, l] = sqrt(omega_j) Theta_lm(chi_j), in double.  With extended-
+    precision chi/omega (theta_grid(n, extended=True)) the table is rounded
+    once: B^T B = I to ~1e-16 with no systematic bias."""
+    return (np.sqrt(omega)[:, None] * assoc_legendre_normalised(lmax, m, chi)).astype(np.float64)
 
 
 def legendre_table(L_H, x):
-    """PL[j, L] = P_L(x_j), L = 0..L_H."""
-    x = np.asarray(x, dtype=np.float64)
-    out = np.zeros((x.size, L_H + 1))
-    out[:, 0] = 1.0
+    """PL[j, L] = P_L(x_j), L = 0..L_H (in the precision of x, returned in double)."""
+    x = np.asarray(x)
+    if x.dtype != EXT:
+        x = x.astype(np.float64)
+    out = np.zeros((x.size, L_H + 1), dtype=x.dtype)
+    out[:, 0] = 1
     if L_H >= 1:
         out[:, 1] = x
     for k in range(1, L_H):
         out[:, k + 1] = ((2 * k + 1) * x * out[:, k] - k * out[:, k - 1]) / (k + 1)
-    return out
+    return out.astype(np.float64)
 
 
 def pair_lists(lmax, m):

@@ -76,10 +76,11 @@ class System:
         self.m = np.array([o[2] for o in self.orbs])
         self.occ = np.array([o[3] for o in self.orbs])
         self.norb = len(self.orbs)
-        self.chi, self.omega = angular.theta_grid(self.n_theta)
-        self.B = {mm: angular.transform_matrix(lmax, mm, self.chi, self.omega)
+        chi_x, omega_x = angular.theta_grid(self.n_theta, extended=True)
+        self.chi, self.omega = chi_x.astype(np.float64), omega_x.astype(np.float64)
+        self.B = {mm: angular.transform_matrix(lmax, mm, chi_x, omega_x)
                   for mm in sorted(set(self.m))}
-        self.PL = angular.legendre_table(self.l_h, self.chi)
+        self.PL = angular.legendre_table(self.l_h, chi_x)
         if gs is None:
             gs = groundstate.ground_state(m, self.Z, self.shells, self.interacting, seed=seed)
         self.gs = gs
