@@ -46,6 +46,7 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 
 #define WG 256
 #define RT 8          // radial nodes per angular tile (16 real columns)
+#define RTD 2         // radial nodes per work-group of k_density / k_dv
 #define RHO_MIN 1e-40 // v_xc = 0 at or below this density
 #define KIN_NARROW_MAX_C 8  // up to 8 complex columns: one 16-wide MFMA column tile
 
@@ -142,7 +143,10 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 
 // Kinetic propagator, pass 2: psi[l][c][i] = sum_a Q[l][i][a] Y[l][c][a].
 // K order is permuted (lane kr of MFMA j in chunk u takes a = k0 + 16u + 4kr + j)
-// so that each lane reads 4 consecutive a of Q and of planar Y.
+// so that each lane reads 4 consecutive a of Q and of planar Y as one vector.
+// Q and Y are zero beyond nkeep up to nk_max (host padding, zeroed Y), so the
+// loop runs to nkeep rounded up to 16 without per-element guards; rows i >= N
+// and columns c >= C read row/column 0 and are not stored.
 #define KU2 4
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
@@ -153,6 +157,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
     typedef typename mfma_t<T>::v4 v4;
     const int l = blockIdx.z;
     const int nk = nkeep[l];
+    const int kend = min(nk_max, (nk + 15) & ~15);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i0 = (blockIdx.x * 4 + wave) * 16;
     if (i0 >= N) return;  // wave-uniform
@@ -161,24 +166,25 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
     const int im_ = i0 + lc;
-    const bool rowok = im_ < N;
-    const T *Ql = Q + (size_t)l * N * nk_max + (size_t)(rowok ? im_ : 0) * nk_max;
-    const T *Yl = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max;
+    const T *Qr = Q + (size_t)l * N * nk_max + (size_t)(im_ < N ? im_ : 0) * nk_max + 4 * kr;
+    const T *Yr = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max + 4 * kr;
     v4 acc = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < nk; k0 += 16 * KU2) {
-        T av[4 * KU2], bv[4 * KU2];
+    for (int k0 = 0; k0 < kend; k0 += 16 * KU2) {
+        v4 av[KU2], bv[KU2];
 #pragma unroll
         for (int u = 0; u < KU2; ++u) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int a = k0 + 16 * u + 4 * kr + j;
-                const bool ok = a < nk;
-                av[4 * u + j] = (ok && rowok) ? Ql[a] : (T)0;
-                bv[4 * u + j] = (ok && colok) ? Yl[a] : (T)0;
+            if (k0 + 16 * u < kend) {
+                av[u] = *reinterpret_cast<const v4 *>(Qr + k0 + 16 * u);
+                bv[u] = *reinterpret_cast<const v4 *>(Yr + k0 + 16 * u);
+            } else {
+                av[u] = (v4){0, 0, 0, 0};
+                bv[u] = (v4){0, 0, 0, 0};
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4 * KU2; ++u) acc = mfma_t<T>::op(av[u], bv[u], acc);
+        for (int u = 0; u < KU2; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc = mfma_t<T>::op(av[u][j], bv[u][j], acc);
     }
     double *Xl = psi + (size_t)l * C * N * 2;
 #pragma unroll
@@ -642,16 +648,22 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
 }
 
 // per-column sums of the observable partials -> obs[step_rel][c][3]
-__global__ void k_obs_reduce(const double *__restrict__ obs_part, double *__restrict__ obs,
-                             int C, int ntiles, const int *cur, const int *origin, int local) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= C * 3) return;
+__global__ __launch_bounds__(64) void k_obs_reduce(const double *__restrict__ obs_part,
+                                                   double *__restrict__ obs, int C, int ntiles,
+                                                   const int *cur, const int *origin, int local) {
+    // one wave per (column, observable); lane t sums tiles t, t+64, ... then a
+    // fixed shuffle tree, so the result does not depend on timing
+    const int idx = blockIdx.x;
     const int c = idx / 3, k = idx % 3;
+    const int lane = threadIdx.x;
     double s = 0.0;
-    for (int tl = 0; tl < ntiles; ++tl) s += obs_part[((size_t)c * ntiles + tl) * 3 + k];
-    // graph replay: the destination row is the step's index within the run call
-    const size_t row = cur ? (size_t)(cur[0] - origin[0] + local) : 0;
-    obs[row * C * 3 + idx] = s;
+    for (int tl = lane; tl < ntiles; tl += 64) s += obs_part[((size_t)c * ntiles + tl) * 3 + k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) {
+        // graph replay: the destination row is the step's index within the run call
+        const size_t row = cur ? (size_t)(cur[0] - origin[0] + local) : 0;
+        obs[row * C * 3 + idx] = s;
+    }
 }
 
 // advances the replayed graph's step counter
@@ -702,10 +714,10 @@ __global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
                                                 const double *__restrict__ PL,
                                                 double *__restrict__ vxc, double *__restrict__ q,
                                                 int N, int Nth, int n_orb, int n_har) {
-    extern __shared__ double rho_s[];  // [RT][Nth]
-    const int i0 = blockIdx.x * RT, inst = blockIdx.y;
+    extern __shared__ double rho_s[];  // [RTD][Nth]
+    const int i0 = blockIdx.x * RTD, inst = blockIdx.y;
     const int t = threadIdx.x;
-    for (int idx = t; idx < RT * Nth; idx += WG) {
+    for (int idx = t; idx < RTD * Nth; idx += WG) {
         const int il = idx / Nth, j = idx % Nth;
         const int i = i0 + il;
         double ru = 0.0;
@@ -719,7 +731,7 @@ __global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
         rho_s[idx] = ru;
     }
     __syncthreads();
-    for (int idx = t; idx < RT * n_har; idx += WG) {
+    for (int idx = t; idx < RTD * n_har; idx += WG) {
         const int il = idx / n_har, L = idx % n_har;
         const int i = i0 + il;
         if (i >= N) continue;
@@ -825,32 +837,34 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
     }
 }
 
-// dV = sum_L P_L(chi_j) V_L(r_i) + v_xc - v_gs; phase = mask e^{-i dV dt/2}
+// dV = sum_L P_L(chi_j) V_L(r_i) + v_xc - v_gs; phase = mask e^{-i dV dt/2}.
+// PLT is the transposed table [L][j] so that the threads of a wave (consecutive
+// j) read consecutive addresses.
 __global__ __launch_bounds__(WG) void k_dv(const double *__restrict__ V,
                                            const double *__restrict__ vxc,
-                                           const double *__restrict__ PL,
+                                           const double *__restrict__ PLT,
                                            const double *__restrict__ v_gs,
                                            const double *__restrict__ mask,
                                            double *__restrict__ dv, double2 *__restrict__ phase,
                                            int N, int Nth, int n_har, double dt, int interacting) {
-    extern __shared__ double Vt[];  // [n_har][RT]
-    const int i0 = blockIdx.x * RT, inst = blockIdx.y;
+    extern __shared__ double Vt[];  // [n_har][RTD]
+    const int i0 = blockIdx.x * RTD, inst = blockIdx.y;
     const int t = threadIdx.x;
     if (interacting) {
-        for (int idx = t; idx < n_har * RT; idx += WG) {
-            const int L = idx / RT, il = idx % RT;
+        for (int idx = t; idx < n_har * RTD; idx += WG) {
+            const int L = idx / RTD, il = idx % RTD;
             const int i = i0 + il;
             Vt[idx] = (i < N) ? V[((size_t)inst * n_har + L) * N + i] : 0.0;
         }
     }
     __syncthreads();
-    for (int idx = t; idx < RT * Nth; idx += WG) {
+    for (int idx = t; idx < RTD * Nth; idx += WG) {
         const int il = idx / Nth, j = idx % Nth;
         const int i = i0 + il;
         if (i >= N) continue;
         double s = 0.0;
         if (interacting) {
-            for (int L = 0; L < n_har; ++L) s += PL[(size_t)j * n_har + L] * Vt[L * RT + il];
+            for (int L = 0; L < n_har; ++L) s += PLT[(size_t)L * Nth + j] * Vt[L * RTD + il];
             s += vxc[((size_t)inst * N + i) * Nth + j] - v_gs[i];
         }
         const size_t o = ((size_t)inst * N + i) * Nth + j;
@@ -893,7 +907,7 @@ struct ggb_ctx {
     int n_field;
     bool field_ring;  // ggb_field_ring: field index = step mod n_field
     // device buffers
-    double *psi, *Y, *Q, *lam, *B, *PL, *chi, *omega, *r, *wr, *v_gs, *mask, *occ, *rho_frozen;
+    double *psi, *Y, *Q, *lam, *B, *PL, *PLT, *chi, *omega, *r, *wr, *v_gs, *mask, *occ, *rho_frozen;
     int32_t *nkeep, *mcol;
     double *dens, *vxc, *q, *V, *dv, *obs_part, *obs, *E;
     double2 *phase;
@@ -951,7 +965,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->p.device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->lam, ctx->B, ctx->PL, ctx->chi, ctx->omega,
+    void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->lam, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
                     ctx->r, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase};
@@ -972,7 +986,7 @@ static size_t footprint_bytes(const ggb_problem *p, int Nlp, int Nthp, int ntile
     const size_t es = p->precision ? 4 : 8;
     size_t tbytes = (Nl * C * nk * 2 + Nl * N * nk + (size_t)p->n_m * Nthp * Nlp) * es;  // Y, Q, B
     d += Nl * nk;                        // lam
-    d += Nth * nh + 2 * Nth;             // PL, chi, omega
+    d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
     d += 5 * N + p->n_orb;               // r, wr, v_gs, mask, rho_frozen, occ
     d += C * N * Nth;                    // dens
     d += 2 * nI * N * Nth;               // vxc, dv
@@ -1020,7 +1034,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->n_field = 0;
     ctx->field_ring = false;
     ctx->obs_cap = 0;
-    ctx->psi = ctx->Y = ctx->Q = ctx->lam = ctx->B = ctx->PL = ctx->chi = ctx->omega = nullptr;
+    ctx->psi = ctx->Y = ctx->Q = ctx->lam = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
     ctx->r = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
     ctx->nkeep = ctx->mcol = nullptr;
     ctx->dens = ctx->vxc = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
@@ -1056,6 +1070,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ok &= dalloc(&ctx->nkeep, Nl) == hipSuccess;
     ok &= dalloc(&ctx->B, (nB * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->PL, Nth * (size_t)p->n_har) == hipSuccess;
+    ok &= dalloc(&ctx->PLT, Nth * (size_t)p->n_har) == hipSuccess;
     ok &= dalloc(&ctx->chi, Nth) == hipSuccess;
     ok &= dalloc(&ctx->omega, Nth) == hipSuccess;
     ok &= dalloc(&ctx->r, N) == hipSuccess;
@@ -1130,6 +1145,12 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
         HIPCHK(ctx, hipMemcpy(ctx->B, b32.data(), b32.size() * 4, hipMemcpyHostToDevice));
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->PL, s->PL, Nth * p.n_har * 8, hipMemcpyHostToDevice, st));
+    {
+        std::vector<double> plt((size_t)Nth * p.n_har);
+        for (size_t j = 0; j < Nth; ++j)
+            for (int L = 0; L < p.n_har; ++L) plt[(size_t)L * Nth + j] = s->PL[j * p.n_har + L];
+        HIPCHK(ctx, hipMemcpy(ctx->PLT, plt.data(), plt.size() * 8, hipMemcpyHostToDevice));
+    }
     HIPCHK(ctx, hipMemcpyAsync(ctx->chi, s->chi, Nth * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->omega, s->omega, Nth * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->mcol, s->m, p.n_orb * 4, hipMemcpyHostToDevice, st));
@@ -1192,19 +1213,19 @@ static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
 static int launch_potential(ggb_ctx *ctx) {
     const ggb_problem &p = ctx->p;
     hipStream_t st = ctx->stream;
-    const dim3 gt(ctx->ntiles, p.n_inst);
+    const dim3 gt((p.n_r + RTD - 1) / RTD, p.n_inst);
     if (p.interacting) {
         if (p.precision)
-            k_density<float><<<gt, WG, (size_t)RT * p.n_theta * 8, st>>>(
+            k_density<float><<<gt, WG, (size_t)RTD * p.n_theta * 8, st>>>(
                 ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PL,
                 ctx->vxc, ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
         else
-            k_density<double><<<gt, WG, (size_t)RT * p.n_theta * 8, st>>>(
+            k_density<double><<<gt, WG, (size_t)RTD * p.n_theta * 8, st>>>(
                 ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PL,
                 ctx->vxc, ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
         k_hartree<<<dim3(p.n_har, p.n_inst), WG, 0, st>>>(ctx->q, ctx->r, ctx->V, p.n_r, p.n_har);
     }
-    k_dv<<<gt, WG, (size_t)p.n_har * RT * 8, st>>>(ctx->V, ctx->vxc, ctx->PL, ctx->v_gs, ctx->mask,
+    k_dv<<<gt, WG, (size_t)p.n_har * RTD * 8, st>>>(ctx->V, ctx->vxc, ctx->PLT, ctx->v_gs, ctx->mask,
                                                    ctx->dv, ctx->phase, p.n_r, p.n_theta, p.n_har,
                                                    p.dt, p.interacting);
     HIPCHK(ctx, hipGetLastError());
@@ -1262,7 +1283,7 @@ static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int
                               Mark between = no_mark) {
     launch_angular<1>(ctx, step, cur);
     if (obs_dst)
-        k_obs_reduce<<<(ctx->C * 3 + 255) / 256, 256, 0, ctx->stream>>>(
+        k_obs_reduce<<<ctx->C * 3, 64, 0, ctx->stream>>>(
             ctx->obs_part, obs_dst, ctx->C, ctx->ntiles, cur, cur ? cur + 1 : nullptr, step);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, between());  // timing event between the angular half and the rebuild
