@@ -99,7 +99,7 @@ def precision_report(sysm, pulse):
             prop.set_field(E)
             o = prop.run(0, n)
         out[prec] = (sysm.total_acceleration(o["acc"][:, 0, :], o["norm"][:, 0, :], E_end),
-                     time.perf_counter() - t0)
+                     time.perf_counter() - t0, o["dip"][:, 0, :] @ sysm.occ)
         log("precision %d full run: %d steps %.1f s" % (prec, n, out[prec][1]))
     a64, a32 = out[0][0], out[1][0]
     rel = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
@@ -112,7 +112,14 @@ def precision_report(sysm, pulse):
     I32 = sp.harmonic_intensities(w, S32, pulse.omega0, orders)
     err = np.log10(I32 / I64)
     plateau = orders < 0.8 * cut
+    d64, d32 = out[0][2], out[1][2]
+    wd, D64 = sp.dipole_spectrum(d64, DT)
+    _, D32 = sp.dipole_spectrum(d32, DT)
+    errd = np.log10(sp.harmonic_intensities(wd, D32, pulse.omega0, orders) /
+                    sp.harmonic_intensities(wd, D64, pulse.omega0, orders))
     return {"steps": n, "acc_rel_l2": rel, "cutoff_order": cut,
+            "dipole_rel_l2": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
+            "dipole_form_log10_error": errd.tolist(),
             "orders": orders.tolist(), "log10_intensity_error": err.tolist(),
             "max_abs_log10_err_plateau": float(np.abs(err[plateau]).max()),
             "max_abs_log10_err_cutoff": float(np.abs(err[~plateau]).max()),
