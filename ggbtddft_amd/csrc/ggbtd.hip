@@ -46,7 +46,6 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 
 #define WG 256
 #define RT 8          // radial nodes per angular tile (16 real columns)
-#define RTD 2         // radial nodes per work-group of k_density / k_dv
 #define RHO_MIN 1e-40 // v_xc = 0 at or below this density
 #define KIN_NARROW_MAX_C 8  // up to 8 complex columns: one 16-wide MFMA column tile
 
@@ -704,6 +703,10 @@ static __device__ T lda_vxc(T rho) {
     return vx + eps - rs / (T)3 * deps;
 }
 
+// Density, LDA v_xc (in T) and multipole moments for one instance and RTM
+// radial nodes: rho_u and v_xc pointwise, then q_L(i) = sum_j P_L(chi_j)
+// rho_u(i, j) as an FP64 matrix-core product (M = L, N = radial node, K = j).
+#define RTM 16
 template <typename T>
 __global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
                                                 const double *__restrict__ occ,
@@ -711,17 +714,18 @@ __global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
                                                 const double *__restrict__ r,
                                                 const double *__restrict__ wr,
                                                 const double *__restrict__ omega,
-                                                const double *__restrict__ PL,
+                                                const double *__restrict__ PLT,
                                                 double *__restrict__ vxc, double *__restrict__ q,
                                                 int N, int Nth, int n_orb, int n_har) {
-    extern __shared__ double rho_s[];  // [RTD][Nth]
-    const int i0 = blockIdx.x * RTD, inst = blockIdx.y;
+    extern __shared__ double rho_s[];  // [Nthp][RTM], zero padded
+    const int i0 = blockIdx.x * RTM, inst = blockIdx.y;
     const int t = threadIdx.x;
-    for (int idx = t; idx < RTD * Nth; idx += WG) {
-        const int il = idx / Nth, j = idx % Nth;
+    const int Nthp = (Nth + 3) & ~3;
+    for (int idx = t; idx < Nthp * RTM; idx += WG) {
+        const int j = idx / RTM, il = idx % RTM;
         const int i = i0 + il;
         double ru = 0.0;
-        if (i < N) {
+        if (i < N && j < Nth) {
             for (int k = 0; k < n_orb; ++k)
                 ru += occ[k] * dens[((size_t)(inst * n_orb + k) * N + i) * Nth + j];
             if (rho_frozen) ru += 0.5 * rho_frozen[i] * omega[j];
@@ -731,13 +735,23 @@ __global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
         rho_s[idx] = ru;
     }
     __syncthreads();
-    for (int idx = t; idx < RTD * n_har; idx += WG) {
-        const int il = idx / n_har, L = idx % n_har;
-        const int i = i0 + il;
-        if (i >= N) continue;
-        double s = 0.0;
-        for (int j = 0; j < Nth; ++j) s += PL[(size_t)j * n_har + L] * rho_s[il * Nth + j];
-        q[((size_t)inst * n_har + L) * N + i] = s;
+    const int wave = t >> 6, lane = t & 63, kr = lane >> 4, lc = lane & 15;
+    const int mt = (n_har + 15) / 16;
+    for (int tile = wave; tile < mt; tile += 4) {
+        const int L = tile * 16 + lc;
+        const double *Arow = PLT + (size_t)(L < n_har ? L : 0) * Nth;
+        d4_t acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < Nthp; k0 += 4) {
+            const int j = k0 + kr;
+            const double a = (L < n_har && j < Nth) ? Arow[j] : 0.0;
+            acc = mfma64(a, rho_s[j * RTM + lc], acc);
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int Lo = tile * 16 + mfma_t<double>::row(kr, rr);
+            const int i = i0 + lc;
+            if (Lo < n_har && i < N) q[((size_t)inst * n_har + Lo) * N + i] = acc[rr];
+        }
     }
 }
 
@@ -746,11 +760,13 @@ __global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
 // scans (ratio form, no overflow):
 //   A_a = A_{a-1} (r_{a-1}/r_a)^{L+1} + q_a / r_a
 //   B_a = (r_a/r_{a+1})^L (B_{a+1} + q_{a+1} / r_{a+1}),   V = A + B
+// with the ratio powers as exp(n lr), lr[a] = log(r[a-1]/r[a]) from the host.
 // One work-group per (L, instance); each thread composes a contiguous chunk,
 // then a Hillis-Steele scan of the chunk maps in LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
                                                 const double *__restrict__ r,
+                                                const double *__restrict__ lr,
                                                 double *__restrict__ V, int N, int n_har) {
     __shared__ double sa[WG], sb[WG];
     const int L = blockIdx.x, inst = blockIdx.y;
@@ -763,7 +779,7 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
     // ---- prefix: A_a = alpha_a A_{a-1} + beta_a
     double al = 1.0, be = 0.0;
     for (int a = s0; a < s1; ++a) {
-        const double alpha = (a > 0) ? pow(r[a - 1] / r[a], (double)(L + 1)) : 0.0;
+        const double alpha = (a > 0) ? exp((double)(L + 1) * lr[a]) : 0.0;
         al *= alpha;
         be = be * alpha + qL[a] / r[a];
     }
@@ -787,7 +803,7 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
     double A = (t > 0) ? sb[t - 1] : 0.0;
     __syncthreads();
     for (int a = s0; a < s1; ++a) {
-        const double alpha = (a > 0) ? pow(r[a - 1] / r[a], (double)(L + 1)) : 0.0;
+        const double alpha = (a > 0) ? exp((double)(L + 1) * lr[a]) : 0.0;
         A = A * alpha + qL[a] / r[a];
         VL[a] = A;
     }
@@ -803,7 +819,7 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
             be = 0.0;
             continue;
         }
-        const double gam = pow(r[a] / r[a + 1], (double)L);
+        const double gam = exp((double)L * lr[a + 1]);
         al *= gam;
         be = be * gam + gam * qL[a + 1] / r[a + 1];
     }
@@ -830,48 +846,57 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
         if (a == N - 1) {
             Bv = 0.0;
         } else {
-            const double gam = pow(r[a] / r[a + 1], (double)L);
+            const double gam = exp((double)L * lr[a + 1]);
             Bv = gam * (Bv + qL[a + 1] / r[a + 1]);
         }
         VL[a] += Bv;
     }
 }
 
-// dV = sum_L P_L(chi_j) V_L(r_i) + v_xc - v_gs; phase = mask e^{-i dV dt/2}.
-// PLT is the transposed table [L][j] so that the threads of a wave (consecutive
-// j) read consecutive addresses.
+// dV = sum_L P_L(chi_j) V_L(r_i) + v_xc - v_gs and phase = mask e^{-i dV dt/2}
+// for one instance and RTM radial nodes; the Legendre sum is an FP64
+// matrix-core product (M = j, N = radial node, K = L).
 __global__ __launch_bounds__(WG) void k_dv(const double *__restrict__ V,
                                            const double *__restrict__ vxc,
-                                           const double *__restrict__ PLT,
+                                           const double *__restrict__ PL,
                                            const double *__restrict__ v_gs,
                                            const double *__restrict__ mask,
                                            double *__restrict__ dv, double2 *__restrict__ phase,
                                            int N, int Nth, int n_har, double dt, int interacting) {
-    extern __shared__ double Vt[];  // [n_har][RTD]
-    const int i0 = blockIdx.x * RTD, inst = blockIdx.y;
+    extern __shared__ double Vt[];  // [n_harp][RTM], zero padded
+    const int i0 = blockIdx.x * RTM, inst = blockIdx.y;
     const int t = threadIdx.x;
-    if (interacting) {
-        for (int idx = t; idx < n_har * RTD; idx += WG) {
-            const int L = idx / RTD, il = idx % RTD;
-            const int i = i0 + il;
-            Vt[idx] = (i < N) ? V[((size_t)inst * n_har + L) * N + i] : 0.0;
-        }
+    const int nhp = (n_har + 3) & ~3;
+    for (int idx = t; idx < nhp * RTM; idx += WG) {
+        const int L = idx / RTM, il = idx % RTM;
+        const int i = i0 + il;
+        Vt[idx] = (interacting && L < n_har && i < N) ? V[((size_t)inst * n_har + L) * N + i] : 0.0;
     }
     __syncthreads();
-    for (int idx = t; idx < RTD * Nth; idx += WG) {
-        const int il = idx / Nth, j = idx % Nth;
-        const int i = i0 + il;
-        if (i >= N) continue;
-        double s = 0.0;
-        if (interacting) {
-            for (int L = 0; L < n_har; ++L) s += PLT[(size_t)L * Nth + j] * Vt[L * RTD + il];
-            s += vxc[((size_t)inst * N + i) * Nth + j] - v_gs[i];
+    const int wave = t >> 6, lane = t & 63, kr = lane >> 4, lc = lane & 15;
+    const int mt = (Nth + 15) / 16;
+    const int i = i0 + lc;
+    for (int tile = wave; tile < mt; tile += 4) {
+        const int j = tile * 16 + lc;
+        const double *Arow = PL + (size_t)(j < Nth ? j : 0) * n_har;
+        d4_t acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < nhp; k0 += 4) {
+            const int L = k0 + kr;
+            const double a = (j < Nth && L < n_har) ? Arow[L] : 0.0;
+            acc = mfma64(a, Vt[L * RTM + lc], acc);
         }
-        const size_t o = ((size_t)inst * N + i) * Nth + j;
-        dv[o] = s;
-        double sn, cs;
-        sincos(0.5 * dt * s, &sn, &cs);
-        phase[o] = make_double2(mask[i] * cs, -mask[i] * sn);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int jo = tile * 16 + mfma_t<double>::row(kr, rr);
+            if (jo >= Nth || i >= N) continue;
+            const size_t o = ((size_t)inst * N + i) * Nth + jo;
+            double sv = 0.0;
+            if (interacting) sv = acc[rr] + vxc[o] - v_gs[i];
+            dv[o] = sv;
+            double sn, cs;
+            sincos(0.5 * dt * sv, &sn, &cs);
+            phase[o] = make_double2(mask[i] * cs, -mask[i] * sn);
+        }
     }
 }
 
@@ -907,7 +932,7 @@ struct ggb_ctx {
     int n_field;
     bool field_ring;  // ggb_field_ring: field index = step mod n_field
     // device buffers
-    double *psi, *Y, *Q, *lam, *B, *PL, *PLT, *chi, *omega, *r, *wr, *v_gs, *mask, *occ, *rho_frozen;
+    double *psi, *Y, *Q, *lam, *B, *PL, *PLT, *chi, *omega, *r, *lr, *wr, *v_gs, *mask, *occ, *rho_frozen;
     int32_t *nkeep, *mcol;
     double *dens, *vxc, *q, *V, *dv, *obs_part, *obs, *E;
     double2 *phase;
@@ -966,7 +991,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     hipSetDevice(ctx->p.device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->lam, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
-                    ctx->r, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
+                    ctx->r, ctx->lr, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase};
     for (void *b : bufs)
@@ -987,7 +1012,7 @@ static size_t footprint_bytes(const ggb_problem *p, int Nlp, int Nthp, int ntile
     size_t tbytes = (Nl * C * nk * 2 + Nl * N * nk + (size_t)p->n_m * Nthp * Nlp) * es;  // Y, Q, B
     d += Nl * nk;                        // lam
     d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
-    d += 5 * N + p->n_orb;               // r, wr, v_gs, mask, rho_frozen, occ
+    d += 6 * N + p->n_orb;               // r, lr, wr, v_gs, mask, rho_frozen, occ
     d += C * N * Nth;                    // dens
     d += 2 * nI * N * Nth;               // vxc, dv
     d += 2 * nI * N * Nth;               // phase (double2)
@@ -1035,7 +1060,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->field_ring = false;
     ctx->obs_cap = 0;
     ctx->psi = ctx->Y = ctx->Q = ctx->lam = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
-    ctx->r = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
+    ctx->r = ctx->lr = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
     ctx->nkeep = ctx->mcol = nullptr;
     ctx->dens = ctx->vxc = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
     ctx->phase = nullptr;
@@ -1074,6 +1099,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ok &= dalloc(&ctx->chi, Nth) == hipSuccess;
     ok &= dalloc(&ctx->omega, Nth) == hipSuccess;
     ok &= dalloc(&ctx->r, N) == hipSuccess;
+    ok &= dalloc(&ctx->lr, N) == hipSuccess;
     ok &= dalloc(&ctx->wr, N) == hipSuccess;
     ok &= dalloc(&ctx->v_gs, N) == hipSuccess;
     ok &= dalloc(&ctx->mask, N) == hipSuccess;
@@ -1118,6 +1144,12 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
     const size_t N = p.n_r, Nl = p.n_l, Nth = p.n_theta;
     hipStream_t st = ctx->stream;
     HIPCHK(ctx, hipMemcpyAsync(ctx->r, s->r, N * 8, hipMemcpyHostToDevice, st));
+    {
+        // lr[a] = log(r[a-1]/r[a]): ratio powers of the Hartree scans as exp(n lr)
+        std::vector<double> lrh(N, 0.0);
+        for (size_t a = 1; a < N; ++a) lrh[a] = std::log(s->r[a - 1] / s->r[a]);
+        HIPCHK(ctx, hipMemcpy(ctx->lr, lrh.data(), N * 8, hipMemcpyHostToDevice));
+    }
     HIPCHK(ctx, hipMemcpyAsync(ctx->wr, s->wr, N * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->v_gs, s->v_gs, N * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->mask, s->mask, N * 8, hipMemcpyHostToDevice, st));
@@ -1213,19 +1245,22 @@ static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
 static int launch_potential(ggb_ctx *ctx) {
     const ggb_problem &p = ctx->p;
     hipStream_t st = ctx->stream;
-    const dim3 gt((p.n_r + RTD - 1) / RTD, p.n_inst);
+    const dim3 gt((p.n_r + RTM - 1) / RTM, p.n_inst);
+    const size_t lds_d = (size_t)((p.n_theta + 3) & ~3) * RTM * 8;
+    const size_t lds_v = (size_t)((p.n_har + 3) & ~3) * RTM * 8;
     if (p.interacting) {
         if (p.precision)
-            k_density<float><<<gt, WG, (size_t)RTD * p.n_theta * 8, st>>>(
-                ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PL,
+            k_density<float><<<gt, WG, lds_d, st>>>(
+                ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PLT,
                 ctx->vxc, ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
         else
-            k_density<double><<<gt, WG, (size_t)RTD * p.n_theta * 8, st>>>(
-                ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PL,
+            k_density<double><<<gt, WG, lds_d, st>>>(
+                ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PLT,
                 ctx->vxc, ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
-        k_hartree<<<dim3(p.n_har, p.n_inst), WG, 0, st>>>(ctx->q, ctx->r, ctx->V, p.n_r, p.n_har);
+        k_hartree<<<dim3(p.n_har, p.n_inst), WG, 0, st>>>(ctx->q, ctx->r, ctx->lr, ctx->V, p.n_r,
+                                                           p.n_har);
     }
-    k_dv<<<gt, WG, (size_t)p.n_har * RTD * 8, st>>>(ctx->V, ctx->vxc, ctx->PLT, ctx->v_gs, ctx->mask,
+    k_dv<<<gt, WG, lds_v, st>>>(ctx->V, ctx->vxc, ctx->PL, ctx->v_gs, ctx->mask,
                                                    ctx->dv, ctx->phase, p.n_r, p.n_theta, p.n_har,
                                                    p.dt, p.interacting);
     HIPCHK(ctx, hipGetLastError());
