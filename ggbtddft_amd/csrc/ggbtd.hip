@@ -81,11 +81,13 @@ static __device__ __forceinline__ double c_lm(int l, int m) {
 // ---------------------------------------------------------------------------
 // Kinetic propagator, pass 1: Y[l][c][p][a] = (e^{-i lam_a dt} sum_i Q[l][i][a] psi[l][c][i])_p
 // M = eigen index a, N = real column (c, p = re/im), K = radial node i.
-// Y is planar in (re, im) so that pass 2 reads contiguous runs of a.
-// Each lane keeps KU loads of A and B in flight before the MFMAs.  T is the
-// compute type (tables and Y in T, state in FP64, phase in FP64).
+// One work-group computes one 16 x 16 output tile; its 4 waves take the four
+// quarters of K (more loads in flight per CU than one wave per tile), then the
+// partial tiles are added in fixed order through LDS.  Y is planar in (re, im)
+// so that pass 2 reads contiguous runs of a.  T is the compute type (tables
+// and Y in T, state in FP64, phase in FP64).
 // ---------------------------------------------------------------------------
-#define KU 16
+#define KU 8
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 const double *__restrict__ lam,
@@ -94,36 +96,48 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 T *__restrict__ Y, int N, int nk_max, int C,
                                                 double dt) {
     typedef typename mfma_t<T>::v4 v4;
+    __shared__ double red[3][64][4];
     const int l = blockIdx.z;
     const int nk = nkeep[l];
+    const int a0 = blockIdx.x * 16;
+    if (a0 >= nk) return;  // block-uniform
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int a0 = (blockIdx.x * 4 + wave) * 16;
-    if (a0 >= nk) return;  // wave-uniform
     const int kr = lane >> 4, lc = lane & 15;
     const int col = blockIdx.y * 16 + lc;
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
     const T *Ql = Q + (size_t)l * N * nk_max + a0 + lc;
     const double *Xl = psi + ((size_t)l * C + (colok ? c : 0)) * N * 2 + p;
+    const int kq = (((N + 3) / 4 + 3) / 4) * 4;  // quarter of K, multiple of 4
+    const int kb = wave * kq, ke = min(N, kb + kq);
     v4 acc = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < N; k0 += 4 * KU) {
+    for (int k0 = kb; k0 < ke; k0 += 4 * KU) {
         T av[KU], bv[KU];
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
             const int i = k0 + 4 * u + kr;
-            const bool ok = i < N;
+            const bool ok = i < ke;
             av[u] = ok ? Ql[(size_t)i * nk_max] : (T)0;
             bv[u] = (ok && colok) ? (T)Xl[(size_t)i * 2] : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < KU; ++u) acc = mfma_t<T>::op(av[u], bv[u], acc);
     }
+    if (wave > 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave - 1][lane][r] = (double)acc[r];
+    __syncthreads();
+    if (wave > 0) return;
+    double v4d[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        v4d[r] = (((double)acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
     const double *laml = lam + (size_t)l * nk_max;
     T o[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int a = a0 + mfma_t<T>::row(kr, r);
-        const double v = (double)acc[r];
+        const double v = v4d[r];
         const double vo = __shfl_xor(v, 1);
         const double re = p ? vo : v, im = p ? v : vo;
         double sn, cs;
@@ -141,12 +155,14 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 }
 
 // Kinetic propagator, pass 2: psi[l][c][i] = sum_a Q[l][i][a] Y[l][c][a].
-// K order is permuted (lane kr of MFMA j in chunk u takes a = k0 + 16u + 4kr + j)
-// so that each lane reads 4 consecutive a of Q and of planar Y as one vector.
-// Q and Y are zero beyond nkeep up to nk_max (host padding, zeroed Y), so the
-// loop runs to nkeep rounded up to 16 without per-element guards; rows i >= N
-// and columns c >= C read row/column 0 and are not stored.
-#define KU2 4
+// One work-group per 16 x 16 output tile, K = a split over the 4 waves as in
+// pass 1.  Within a wave the K order is permuted (lane kr of MFMA j in chunk u
+// takes a = k0 + 16u + 4kr + j) so that each lane reads 4 consecutive a of Q
+// and of planar Y as one vector.  Q and Y are zero beyond nkeep up to nk_max
+// (host padding, zeroed Y), so the loop runs to nkeep rounded up to 16
+// without per-element guards; rows i >= N and columns c >= C read row/column
+// 0 and are not stored.
+#define KU2 2
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
                                                 const int32_t *__restrict__ nkeep,
@@ -154,12 +170,12 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
                                                 double *__restrict__ psi, int N, int nk_max,
                                                 int C) {
     typedef typename mfma_t<T>::v4 v4;
+    __shared__ double red[3][64][4];
     const int l = blockIdx.z;
     const int nk = nkeep[l];
     const int kend = min(nk_max, (nk + 15) & ~15);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i0 = (blockIdx.x * 4 + wave) * 16;
-    if (i0 >= N) return;  // wave-uniform
+    const int i0 = blockIdx.x * 16;
     const int kr = lane >> 4, lc = lane & 15;
     const int col = blockIdx.y * 16 + lc;
     const int c = col >> 1, p = col & 1;
@@ -167,12 +183,14 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
     const int im_ = i0 + lc;
     const T *Qr = Q + (size_t)l * N * nk_max + (size_t)(im_ < N ? im_ : 0) * nk_max + 4 * kr;
     const T *Yr = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max + 4 * kr;
+    const int kq = (((kend + 15) / 16 + 3) / 4) * 16;  // quarter of K, multiple of 16
+    const int kb = wave * kq, ke = min(kend, kb + kq);
     v4 acc = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < kend; k0 += 16 * KU2) {
+    for (int k0 = kb; k0 < ke; k0 += 16 * KU2) {
         v4 av[KU2], bv[KU2];
 #pragma unroll
         for (int u = 0; u < KU2; ++u) {
-            if (k0 + 16 * u < kend) {
+            if (k0 + 16 * u < ke) {
                 av[u] = *reinterpret_cast<const v4 *>(Qr + k0 + 16 * u);
                 bv[u] = *reinterpret_cast<const v4 *>(Yr + k0 + 16 * u);
             } else {
@@ -185,11 +203,17 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc = mfma_t<T>::op(av[u][j], bv[u][j], acc);
     }
+    if (wave > 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave - 1][lane][r] = (double)acc[r];
+    __syncthreads();
+    if (wave > 0) return;
     double *Xl = psi + (size_t)l * C * N * 2;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = i0 + mfma_t<T>::row(kr, r);
-        if (colok && i < N) Xl[((size_t)c * N + i) * 2 + p] = (double)acc[r];
+        const double v = (((double)acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+        if (colok && i < N) Xl[((size_t)c * N + i) * 2 + p] = v;
     }
 }
 
@@ -1290,10 +1314,10 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
     }
     const int ncol = (2 * ctx->C + 15) / 16;
     if (pass == 0)
-        k_kin_fwd<T><<<dim3((p.nk_max + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
+        k_kin_fwd<T><<<dim3(p.nk_max / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
             Q, ctx->lam, ctx->nkeep, ctx->psi, Y, p.n_r, p.nk_max, ctx->C, p.dt);
     else
-        k_kin_bwd<T><<<dim3((p.n_r + 63) / 64, ncol, p.n_l), WG, 0, ctx->stream>>>(
+        k_kin_bwd<T><<<dim3((p.n_r + 15) / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
             Q, ctx->nkeep, Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
 }
 

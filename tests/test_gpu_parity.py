@@ -133,12 +133,15 @@ def test_batched_instances_match_alone():
             prop.set_field(fields[k])
             alone = prop.run(0, nsteps)
             psi_alone = prop.download_orbitals()
-        # the batched context uses the tiled kinetic kernels, the single one the
-        # narrow kernels; both sum in the same order, so the results are equal
+        # the batched context uses the tiled kinetic kernels (one wave runs
+        # the whole contraction), the single one the narrow kernels (the
+        # contraction split over four waves): the summation order differs, so
+        # results agree to rounding, 1e-12 of each quantity's scale
         for key in ("dip", "acc", "norm"):
-            assert np.array_equal(mix[key][:, k, :], alone[key][:, 0, :]), (k, key)
+            d = np.abs(mix[key][:, k, :] - alone[key][:, 0, :]).max()
+            assert d <= 1e-12 * np.abs(alone[key]).max(), (k, key, d)
         cols = slice(k * S.n_orb, (k + 1) * S.n_orb)
-        assert np.array_equal(psi_mix[:, cols, :], psi_alone)
+        assert np.abs(psi_mix[:, cols, :] - psi_alone).max() < 1e-12
 
 
 @pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30)])
