@@ -33,6 +33,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -421,16 +422,32 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
 
 // ---------------------------------------------------------------------------
 // Angular kernel.  One work-group = one orbital column c, RT radial nodes.
-// LDS holds the l-space tile X[Nlp][16] and the theta-space tile P[Nthp][16]
-// (real column n = 2 * i_local + re/im) in the compute type T; the state in
-// HBM, the phase factors and the observable sums stay FP64.
+//
+// Gauss-Legendre nodes are symmetric, chi_{j'} = -chi_j with j' = Nth-1-j, and
+// Theta_lm(-x) = (-1)^{l-m} Theta_lm(x).  Splitting the l of an orbital by the
+// parity of l - m (s: l = m + 2k, a: l = m + 1 + 2k) and keeping the first
+// h = ceil(Nth/2) nodes, the transform of the tile X[l][col] becomes
+//     E = Bs X_s,  O = Ba X_a,   Psi_j = E_j + O_j,  Psi_{j'} = E_j - O_j,
+// and the back transform of the theta-space values P
+//     X_s = Bs^T S,  X_a = Ba^T D,  S_j = P_j + P_{j'},  D_j = P_j - P_{j'}
+// (middle node of odd Nth: S = D = P_mid).  That is half the matrix-core work
+// of the dense N_theta x N_l transform, and the two half tables of one m
+// (hq x (sS + sA) values) are staged in LDS once per work-group (GLB = true
+// reads them from global memory when they do not fit).
+// LDS: tables | X[Nl][16] | S[hq][16] | D[hq][16] in the compute type T (real
+// column n = 2 * i_local + re/im); the state in HBM, the phase factors and
+// the observable sums stay FP64.
 // mode 0: Vh, Le, Lo          (first half of the step)
 // mode 1: Lo, Le, Vh, store, then |B psi|^2 + observable partials
 // mode 2: |B psi|^2 + observable partials only (initial potential)
+// Waves: wave w takes theta tile w of the forward product and output tile w
+// of the back product (s tiles first, then a tiles); the launch uses
+// max(ceil(h/16), ceil(ns/16) + ceil(na/16)) <= 8 waves.
 // ---------------------------------------------------------------------------
+#define ANG_MAXW 8
 struct AngArgs {
     double *psi;
-    const void *B;         // [n_m][Nthp][Nlp] zero padded, compute type
+    const void *Bh;        // [n_m][nblk] parity half tables, compute type
     const int32_t *mcol;   // [n_orb]
     const double2 *phase;  // [n_inst][N][Nth] mask e^{-i dV dt/2}
     const double *r;
@@ -439,7 +456,7 @@ struct AngArgs {
     double *dens;          // [C][N][Nth] |Psi|^2
     double *obs_part;      // [C][ntiles][3]
     const int *cur;        // graph replay: field step = (cur[0] + step) mod n_field
-    int N, Nl, Nlp, Nth, Nthp, C, n_orb, n_field, step;
+    int N, Nl, Nlx, Nth, h, hq, sS, sA, nblk, C, n_orb, n_field, step;
     double dt;
 };
 
@@ -448,7 +465,7 @@ static __device__ void ang_laser(T *X, int parity, int m, int Nl, const double *
     // pairs (l, l+1), l = m + parity + 2 q, l + 1 < Nl; thread -> (q, i_local)
     const int t = threadIdx.x;
     const int il = t & 7;
-    for (int q = t >> 3;; q += WG / 8) {
+    for (int q = t >> 3;; q += blockDim.x / 8) {
         const int l = m + parity + 2 * q;
         if (l + 1 >= Nl) break;
         const double th = a_i[il] * c_lm(l, m);
@@ -466,92 +483,126 @@ static __device__ void ang_laser(T *X, int parity, int m, int Nl, const double *
     }
 }
 
-// theta-space product P = B X; accumulators for up to 4 M-tiles per wave.
-// K order permuted (lane kr of MFMA j takes l = k0 + 4kr + j) so that each
-// lane reads 4 consecutive l of its row of B; 8 loads in flight per lane.
+// parity sizes of orbital m: ns = #{l = m + 2k <= Nl-1}, na = #{l = m + 1 + 2k <= Nl-1}
+struct ang_par {
+    int ns, na, ks, ka;
+    __device__ ang_par(int Nl, int m) {
+        ns = (Nl - 1 - m) / 2 + 1;
+        na = (Nl - m) / 2;
+        ks = (ns + 3) & ~3;
+        ka = (na + 3) & ~3;
+    }
+};
+
+// forward product of theta tile `tile` (rows j = 16 tile + 0..15 of the first
+// half): e = Bs X_s, o = Ba X_a; two independent accumulator chains.
 template <typename T>
-static __device__ void ang_fwd(const T *Bm, const T *X, int Nlp, int Nthp,
-                               typename mfma_t<T>::v4 *acc) {
+static __device__ void ang_fwd(const T *Bs, const T *Ba, const T *X, const AngArgs &g,
+                               const ang_par &pr, int m, int tile, typename mfma_t<T>::v4 &e,
+                               typename mfma_t<T>::v4 &o) {
     typedef typename mfma_t<T>::v4 v4;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int kr = lane >> 4, lc = lane & 15;
-    const int mt = Nthp / 16;
-    for (int t = 0; t < 4; ++t) {
-        acc[t] = (v4){0, 0, 0, 0};
-        const int tile = wave + 4 * t;
-        if (tile >= mt) continue;
-        const T *Arow = Bm + (size_t)(tile * 16 + lc) * Nlp + 4 * kr;
-        v4 a = acc[t];
-        for (int k0 = 0; k0 < Nlp; k0 += 32) {
-            T av[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int kb = k0 + 16 * (u >> 2) + (u & 3);
-                av[u] = (kb < Nlp) ? Arow[kb] : (T)0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int kb = k0 + 16 * (u >> 2) + (u & 3);
-                if (kb < Nlp) a = mfma_t<T>::op(av[u], X[(kb + 4 * kr) * 16 + lc], a);
-            }
+    const int lane = threadIdx.x & 63, kr = lane >> 4, lc = lane & 15;
+    const int j = tile * 16 + lc;
+    const bool jok = j < g.hq;
+    const T *As = Bs + (size_t)(jok ? j : 0) * g.sS;
+    const T *Aa = Ba + (size_t)(jok ? j : 0) * g.sA;
+    v4 ae = (v4){0, 0, 0, 0}, ao = (v4){0, 0, 0, 0};
+    const int kmax = pr.ks > pr.ka ? pr.ks : pr.ka;
+    for (int k0 = 0; k0 < kmax; k0 += 4) {
+        const int k = k0 + kr;
+        if (k0 < pr.ks) {
+            const int l = m + 2 * k;
+            const T a = jok ? As[k] : (T)0;
+            const T b = l < g.Nl ? X[l * 16 + lc] : (T)0;
+            ae = mfma_t<T>::op(a, b, ae);
         }
-        acc[t] = a;
+        if (k0 < pr.ka) {
+            const int l = m + 1 + 2 * k;
+            const T a = jok ? Aa[k] : (T)0;
+            const T b = l < g.Nl ? X[l * 16 + lc] : (T)0;
+            ao = mfma_t<T>::op(a, b, ao);
+        }
+    }
+    e = ae;
+    o = ao;
+}
+
+// back product of output tile `item` (s tiles, then a tiles) into X; two
+// accumulator chains over alternating groups of 4 nodes, added at the end.
+template <typename T>
+static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D, T *X,
+                               const AngArgs &g, const ang_par &pr, int m, int item) {
+    typedef typename mfma_t<T>::v4 v4;
+    const int lane = threadIdx.x & 63, kr = lane >> 4, lc = lane & 15;
+    const int nst = (pr.ns + 15) / 16;
+    const bool sym = item < nst;
+    const int tile = sym ? item : item - nst;
+    const T *A = sym ? Bs : Ba;
+    const T *Pv = sym ? S : D;
+    const int st = sym ? g.sS : g.sA;
+    const int kk = sym ? pr.ks : pr.ka;
+    const int nn = sym ? pr.ns : pr.na;
+    const int k = tile * 16 + lc;
+    const bool kok = k < kk;
+    const T *Ac = A + (kok ? k : 0);
+    v4 a0 = (v4){0, 0, 0, 0}, a1 = (v4){0, 0, 0, 0};
+    int j0 = 0;
+    for (; j0 + 8 <= g.hq; j0 += 8) {
+        const int ja = j0 + kr, jb = j0 + 4 + kr;
+        const T va = kok ? Ac[(size_t)ja * st] : (T)0;
+        const T vb = kok ? Ac[(size_t)jb * st] : (T)0;
+        a0 = mfma_t<T>::op(va, Pv[ja * 16 + lc], a0);
+        a1 = mfma_t<T>::op(vb, Pv[jb * 16 + lc], a1);
+    }
+    if (j0 < g.hq) {
+        const int ja = j0 + kr;
+        const T va = kok ? Ac[(size_t)ja * st] : (T)0;
+        a0 = mfma_t<T>::op(va, Pv[ja * 16 + lc], a0);
+    }
+    const int off = sym ? 0 : 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int ko = tile * 16 + mfma_t<T>::row(kr, r);
+        if (ko < nn) X[(m + off + 2 * ko) * 16 + lc] = a0[r] + a1[r];
     }
 }
 
-// l-space product X = B^T P; up to 2 M-tiles per wave, 8 loads in flight.
-template <typename T>
-static __device__ void ang_bwd(const T *Bm, const T *P, int Nlp, int Nthp,
-                               typename mfma_t<T>::v4 *acc) {
+template <int MODE, typename T, bool GLB>
+__global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     typedef typename mfma_t<T>::v4 v4;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int kr = lane >> 4, lc = lane & 15;
-    const int mt = Nlp / 16;
-    for (int t = 0; t < 2; ++t) {
-        acc[t] = (v4){0, 0, 0, 0};
-        const int tile = wave + 4 * t;
-        if (tile >= mt) continue;
-        const T *Acol = Bm + (size_t)kr * Nlp + tile * 16 + lc;
-        v4 a = acc[t];
-        for (int k0 = 0; k0 < Nthp; k0 += 32) {
-            T av[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + 4 * u;
-                av[u] = (k < Nthp) ? Acol[(size_t)k * Nlp] : (T)0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + 4 * u;
-                if (k < Nthp) a = mfma_t<T>::op(av[u], P[(k + kr) * 16 + lc], a);
-            }
-        }
-        acc[t] = a;
-    }
-}
-
-template <int MODE, typename T>
-__global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
-    typedef typename mfma_t<T>::v4 v4;
-    extern __shared__ double lds[];
-    const int Nlp = g.Nlp, Nthp = g.Nthp;
-    T *X = reinterpret_cast<T *>(lds);  // [Nlp][16]
-    T *P = X + Nlp * 16;                // [Nthp][16]
+    extern __shared__ __align__(16) double lds[];
     __shared__ double a_i[RT];
-    __shared__ double red[3][WG / 64];
+    __shared__ double red[3][ANG_MAXW];
 
     const int tile_i = blockIdx.x, c = blockIdx.y;
     const int i0 = tile_i * RT;
     const int inst = c / g.n_orb;
     const int m = g.mcol[c % g.n_orb];
-    const T *Bm = reinterpret_cast<const T *>(g.B) + (size_t)m * Nthp * Nlp;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, nthr = blockDim.x;
     const int wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int ilc = lc >> 1, p = lc & 1;  // column -> (i_local, re/im)
+    const ang_par pr(g.Nl, m);
+    const int mt = (g.h + 15) / 16;
+    const int nbt = (pr.ns + 15) / 16 + (pr.na + 15) / 16;
 
+    const T *Bsrc = reinterpret_cast<const T *>(g.Bh) + (size_t)m * g.nblk;
+    T *Tl = reinterpret_cast<T *>(lds);
+    T *X = GLB ? Tl : Tl + g.nblk;  // [Nlx][16]
+    T *S = X + g.Nlx * 16;          // [hq][16]
+    T *D = S + g.hq * 16;           // [hq][16]
+    const T *Bs = GLB ? Bsrc : Tl;
+    const T *Ba = Bs + (size_t)g.hq * g.sS;
+
+    if (!GLB) {
+        // the block of one m is a multiple of 16 bytes (host pads nblk)
+        const int n16 = g.nblk * (int)sizeof(T) / 16;
+        const int4 *s4 = reinterpret_cast<const int4 *>(Bsrc);
+        int4 *d4 = reinterpret_cast<int4 *>(Tl);
+        for (int idx = t; idx < n16; idx += nthr) d4[idx] = s4[idx];
+    }
     // load the l-space tile
-    for (int idx = t; idx < Nlp * RT; idx += WG) {
+    for (int idx = t; idx < g.Nlx * RT; idx += nthr) {
         const int l = idx / RT, il = idx % RT;
         const int i = i0 + il;
         double2 v = make_double2(0.0, 0.0);
@@ -574,38 +625,38 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
         __syncthreads();
     }
 
-    v4 acc[4];
+    const int i = i0 + ilc;
+    const bool iok = i < g.N;
     if (MODE != 2) {
-        // Vh: theta space, phase, back to l space
-        ang_fwd<T>(Bm, X, Nlp, Nthp, acc);
-        const double2 *ph = g.phase + (size_t)inst * g.N * g.Nth;
-        const int i = i0 + ilc;
-        for (int tt = 0; tt < 4; ++tt) {
-            const int tile = wave + 4 * tt;
-            if (tile >= Nthp / 16) continue;
+        // Vh: theta space, phase, folded sums S/D, back to l space
+        if (wave < mt) {
+            v4 e, o;
+            ang_fwd<T>(Bs, Ba, X, g, pr, m, wave, e, o);
+            const double2 *ph = g.phase + ((size_t)inst * g.N + (iok ? i : 0)) * g.Nth;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int j = tile * 16 + mfma_t<T>::row(kr, r);
-                const T v = acc[tt][r];
-                const T vo = __shfl_xor(v, 1);
-                T o = (T)0;
-                if (j < g.Nth && i < g.N) {
-                    const double2 f = ph[(size_t)i * g.Nth + j];
-                    const T fx = (T)f.x, fy = (T)f.y;
-                    const T re = p ? vo : v, im = p ? v : vo;
-                    o = p ? (re * fy + im * fx) : (re * fx - im * fy);
+                const int j = wave * 16 + mfma_t<T>::row(kr, r);
+                const int jm = g.Nth - 1 - j;
+                const T pa = e[r] + o[r], pb = e[r] - o[r];
+                const T pao = __shfl_xor(pa, 1), pbo = __shfl_xor(pb, 1);
+                T sv = (T)0, dv = (T)0;
+                if (j < g.h && iok) {
+                    const double2 fa = ph[j], fb = ph[jm];
+                    const T ra = p ? pao : pa, ia = p ? pa : pao;
+                    const T rb = p ? pbo : pb, ib = p ? pb : pbo;
+                    const T qa = p ? (ra * (T)fa.y + ia * (T)fa.x) : (ra * (T)fa.x - ia * (T)fa.y);
+                    const T qb = p ? (rb * (T)fb.y + ib * (T)fb.x) : (rb * (T)fb.x - ib * (T)fb.y);
+                    sv = (jm == j) ? qa : qa + qb;
+                    dv = (jm == j) ? qa : qa - qb;
                 }
-                P[j * 16 + lc] = o;
+                if (j < g.hq) {
+                    S[j * 16 + lc] = sv;
+                    D[j * 16 + lc] = dv;
+                }
             }
         }
         __syncthreads();
-        ang_bwd<T>(Bm, P, Nlp, Nthp, acc);
-        for (int tt = 0; tt < 2; ++tt) {
-            const int tile = wave + 4 * tt;
-            if (tile >= Nlp / 16) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) X[(tile * 16 + mfma_t<T>::row(kr, r)) * 16 + lc] = acc[tt][r];
-        }
+        if (wave < nbt) ang_bwd<T>(Bs, Ba, S, D, X, g, pr, m, wave);
         __syncthreads();
     }
 
@@ -617,37 +668,43 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
     }
 
     if (MODE != 2) {
-        for (int idx = t; idx < g.Nl * RT; idx += WG) {
+        for (int idx = t; idx < g.Nl * RT; idx += nthr) {
             const int l = idx / RT, il = idx % RT;
-            const int i = i0 + il;
-            if (i < g.N)
-                ((double2 *)g.psi)[((size_t)l * g.C + c) * g.N + i] =
+            const int ii = i0 + il;
+            if (ii < g.N)
+                ((double2 *)g.psi)[((size_t)l * g.C + c) * g.N + ii] =
                     make_double2((double)X[l * 16 + 2 * il], (double)X[l * 16 + 2 * il + 1]);
         }
     }
     if (MODE == 0) return;
 
     // |B psi|^2 and observable partials (accumulated in FP64)
-    ang_fwd<T>(Bm, X, Nlp, Nthp, acc);
-    const int i = i0 + ilc;
-    const bool iok = i < g.N;
     const double ri = iok ? g.r[i] : 1.0;
     double sd = 0.0, sa = 0.0, sn = 0.0;
-    double *dens = g.dens + (size_t)c * g.N * g.Nth;
-    for (int tt = 0; tt < 4; ++tt) {
-        const int tile = wave + 4 * tt;
-        if (tile >= Nthp / 16) continue;
+    if (wave < mt) {
+        v4 e, o;
+        ang_fwd<T>(Bs, Ba, X, g, pr, m, wave, e, o);
+        double *dens = g.dens + ((size_t)c * g.N + (iok ? i : 0)) * g.Nth;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int j = tile * 16 + mfma_t<T>::row(kr, r);
-            const double v = (double)acc[tt][r];
-            const double s2 = v * v + __shfl_xor(v * v, 1);
-            if (p == 0 && iok && j < g.Nth) {
-                dens[(size_t)i * g.Nth + j] = s2;
-                const double cj = g.chi[j];
-                sd += ri * cj * s2;
-                sa += cj * s2 / (ri * ri);
-                sn += s2;
+            const int j = wave * 16 + mfma_t<T>::row(kr, r);
+            const int jm = g.Nth - 1 - j;
+            const double va = (double)(e[r] + o[r]), vb = (double)(e[r] - o[r]);
+            const double s2a = va * va + __shfl_xor(va * va, 1);
+            const double s2b = vb * vb + __shfl_xor(vb * vb, 1);
+            if (p == 0 && iok && j < g.h) {
+                const double ca = g.chi[j];
+                dens[j] = s2a;
+                sd += ri * ca * s2a;
+                sa += ca * s2a / (ri * ri);
+                sn += s2a;
+                if (jm != j) {
+                    const double cb = g.chi[jm];
+                    dens[jm] = s2b;
+                    sd += ri * cb * s2b;
+                    sa += cb * s2b / (ri * ri);
+                    sn += s2b;
+                }
             }
         }
     }
@@ -665,7 +722,7 @@ __global__ __launch_bounds__(WG) void k_angular(AngArgs g) {
     __syncthreads();
     if (t < 3) {
         double s = 0.0;
-        for (int w = 0; w < WG / 64; ++w) s += red[t][w];
+        for (int w = 0; w < nthr / 64; ++w) s += red[t][w];
         g.obs_part[((size_t)c * gridDim.x + tile_i) * 3 + t] = s;
     }
 }
@@ -947,9 +1004,37 @@ __global__ void k_selftest_mfma(double *out) {
 // ===========================================================================
 // Host side
 // ===========================================================================
+// Sizes of the parity-split angular tables and the angular kernel's launch.
+#define ANG_LDS_MAX 65536  // tables staged in LDS when everything fits in 64 KiB
+struct ang_geom {
+    int h, hq, sS, sA, nblk, waves;
+    size_t lds;  // dynamic LDS bytes
+    bool glb;    // tables read from global memory
+};
+static ang_geom ang_geometry(const ggb_problem *p) {
+    ang_geom a;
+    const int es = p->precision ? 4 : 8;
+    const int L = p->n_l - 1;
+    const int ns0 = L / 2 + 1, na0 = (L + 1) / 2;  // m = 0 has the most l of each parity
+    a.h = (p->n_theta + 1) / 2;
+    a.hq = (a.h + 3) & ~3;
+    a.sS = ((ns0 + 3) & ~3) + 1;  // odd row strides spread a tile's rows over the LDS banks
+    a.sA = ((na0 + 3) & ~3) + 1;
+    const int per16 = 16 / es;
+    a.nblk = ((a.hq * (a.sS + a.sA) + per16 - 1) / per16) * per16;
+    const int mt = (a.h + 15) / 16, nb = (ns0 + 15) / 16 + (na0 + 15) / 16;
+    a.waves = mt > nb ? mt : nb;
+    const size_t tiles = (size_t)(p->n_l * 16 + 2 * a.hq * 16) * es;
+    const size_t stat = (RT + 3 * ANG_MAXW) * sizeof(double);
+    a.glb = tiles + (size_t)a.nblk * es + stat > ANG_LDS_MAX;
+    a.lds = a.glb ? tiles : tiles + (size_t)a.nblk * es;
+    return a;
+}
+
 struct ggb_ctx {
     ggb_problem p;
-    int Nlp, Nthp, C, ntiles;
+    int C, ntiles;
+    ang_geom ang;  // parity-split angular tables and launch
     hipStream_t stream;
     std::string err;
     bool have_static, have_orbitals;
@@ -1027,13 +1112,14 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     delete ctx;
 }
 
-static size_t footprint_bytes(const ggb_problem *p, int Nlp, int Nthp, int ntiles) {
+static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     const size_t N = p->n_r, Nl = p->n_l, C = (size_t)p->n_orb * p->n_inst, Nth = p->n_theta;
     const size_t nI = p->n_inst, nk = p->nk_max, nh = p->n_har;
     size_t d = 0;  // doubles
     d += Nl * C * N * 2;                 // psi
     const size_t es = p->precision ? 4 : 8;
-    size_t tbytes = (Nl * C * nk * 2 + Nl * N * nk + (size_t)p->n_m * Nthp * Nlp) * es;  // Y, Q, B
+    const size_t nblk = ang_geometry(p).nblk;
+    size_t tbytes = (Nl * C * nk * 2 + Nl * N * nk + (size_t)p->n_m * nblk) * es;  // Y, Q, B
     d += Nl * nk;                        // lam
     d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
     d += 6 * N + p->n_orb;               // r, lr, wr, v_gs, mask, rho_frozen, occ
@@ -1048,8 +1134,7 @@ static size_t footprint_bytes(const ggb_problem *p, int Nlp, int Nthp, int ntile
 extern "C" int ggb_footprint(const ggb_problem *p, uint64_t *bytes) {
     if (!p || !bytes || p->n_r < 1 || p->n_l < 1 || p->n_theta < 1 || p->n_orb < 1 || p->n_inst < 1)
         return GGB_E_ARG;
-    const int Nlp = ((p->n_l + 15) / 16) * 16, Nthp = ((p->n_theta + 15) / 16) * 16;
-    *bytes = footprint_bytes(p, Nlp, Nthp, (p->n_r + RT - 1) / RT);
+    *bytes = footprint_bytes(p, (p->n_r + RT - 1) / RT);
     return GGB_OK;
 }
 
@@ -1075,8 +1160,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ggb_ctx *ctx = new ggb_ctx();
     memset((void *)&ctx->p, 0, sizeof(ctx->p));
     ctx->p = *p;
-    ctx->Nlp = ((p->n_l + 15) / 16) * 16;
-    ctx->Nthp = ((p->n_theta + 15) / 16) * 16;
+    ctx->ang = ang_geometry(p);
     ctx->C = p->n_orb * p->n_inst;
     ctx->ntiles = (p->n_r + RT - 1) / RT;
     ctx->have_static = ctx->have_orbitals = false;
@@ -1109,10 +1193,10 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     const size_t N = p->n_r, Nl = p->n_l, C = ctx->C, Nth = p->n_theta, nI = p->n_inst;
     bool ok = true;
     ok &= dalloc(&ctx->psi, Nl * C * N * 2) == hipSuccess;
-    // Y, Q and B hold the compute type (8 or 4 bytes per element)
+    // Y, Q and the angular tables hold the compute type (8 or 4 bytes per element)
     const size_t es = p->precision ? sizeof(float) : sizeof(double);
     const size_t nY = Nl * C * (size_t)p->nk_max * 2, nQ = Nl * N * (size_t)p->nk_max;
-    const size_t nB = (size_t)p->n_m * ctx->Nthp * ctx->Nlp;
+    const size_t nB = (size_t)p->n_m * ctx->ang.nblk;
     ok &= dalloc(&ctx->Y, (nY * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->Q, (nQ * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->lam, Nl * (size_t)p->nk_max) == hipSuccess;
@@ -1186,19 +1270,46 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->lam, s->lam, Nl * p.nk_max * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->nkeep, s->nkeep, Nl * 4, hipMemcpyHostToDevice, st));
-    // B is stored zero padded to [n_m][Nthp][Nlp]
-    if (p.precision == 0) {
-        for (int mm = 0; mm < p.n_m; ++mm)
-            HIPCHK(ctx, hipMemcpy2DAsync(ctx->B + (size_t)mm * ctx->Nthp * ctx->Nlp, ctx->Nlp * 8,
-                                         s->B + (size_t)mm * Nth * Nl, Nl * 8, Nl * 8, Nth,
-                                         hipMemcpyHostToDevice, st));
-    } else {
-        std::vector<float> b32((size_t)p.n_m * ctx->Nthp * ctx->Nlp, 0.f);
+    {
+        // parity half tables of each m (see the angular kernel): rows j < h of
+        // the theta grid, Bs[j][k] = B[m][j][m + 2k], Ba[j][k] = B[m][j][m + 1 + 2k],
+        // zero padded.  They stand for the whole grid only if it is symmetric.
+        const ang_geom &ag = ctx->ang;
+        double bmax = 0.0;
+        for (size_t k = 0; k < (size_t)p.n_m * Nth * Nl; ++k) bmax = std::max(bmax, std::fabs(s->B[k]));
+        const double tol = 1e-13 * (bmax > 0.0 ? bmax : 1.0);
+        for (size_t j = 0; j < Nth; ++j)
+            if (std::fabs(s->chi[j] + s->chi[Nth - 1 - j]) > 1e-13 ||
+                std::fabs(s->omega[j] - s->omega[Nth - 1 - j]) > 1e-13 * std::fabs(s->omega[j]))
+                return fail(ctx, GGB_E_ARG, "ggb_upload_static: theta grid must be symmetric (Gauss-Legendre)");
         for (int mm = 0; mm < p.n_m; ++mm)
             for (size_t j = 0; j < Nth; ++j)
-                for (size_t l = 0; l < Nl; ++l)
-                    b32[((size_t)mm * ctx->Nthp + j) * ctx->Nlp + l] = (float)s->B[((size_t)mm * Nth + j) * Nl + l];
-        HIPCHK(ctx, hipMemcpy(ctx->B, b32.data(), b32.size() * 4, hipMemcpyHostToDevice));
+                for (size_t l = 0; l < Nl; ++l) {
+                    const double sg = ((l + mm) & 1) ? -1.0 : 1.0;
+                    const double a = s->B[((size_t)mm * Nth + j) * Nl + l];
+                    const double b = s->B[((size_t)mm * Nth + Nth - 1 - j) * Nl + l];
+                    if (std::fabs(a - sg * b) > tol || ((int)l < mm && a != 0.0))
+                        return fail(ctx, GGB_E_ARG,
+                                    "ggb_upload_static: B must satisfy B[m][N_theta-1-j][l] = (-1)^(l-m) B[m][j][l]"
+                                    " and vanish for l < m");
+                }
+        std::vector<double> bh((size_t)p.n_m * ag.nblk, 0.0);
+        for (int mm = 0; mm < p.n_m; ++mm) {
+            double *Bs = bh.data() + (size_t)mm * ag.nblk;
+            double *Ba = Bs + (size_t)ag.hq * ag.sS;
+            for (int j = 0; j < ag.h; ++j) {
+                const double *row = s->B + ((size_t)mm * Nth + j) * Nl;
+                for (int k = 0; mm + 2 * k < (int)Nl; ++k) Bs[(size_t)j * ag.sS + k] = row[mm + 2 * k];
+                for (int k = 0; mm + 1 + 2 * k < (int)Nl; ++k) Ba[(size_t)j * ag.sA + k] = row[mm + 1 + 2 * k];
+            }
+        }
+        if (p.precision == 0) {
+            HIPCHK(ctx, hipMemcpy(ctx->B, bh.data(), bh.size() * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> b32(bh.size());
+            for (size_t k = 0; k < bh.size(); ++k) b32[k] = (float)bh[k];
+            HIPCHK(ctx, hipMemcpy(ctx->B, b32.data(), b32.size() * 4, hipMemcpyHostToDevice));
+        }
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->PL, s->PL, Nth * p.n_har * 8, hipMemcpyHostToDevice, st));
     {
@@ -1228,7 +1339,7 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
 static AngArgs ang_args(ggb_ctx *ctx, int step) {
     AngArgs g;
     g.psi = ctx->psi;
-    g.B = ctx->B;
+    g.Bh = ctx->B;
     g.mcol = ctx->mcol;
     g.phase = ctx->phase;
     g.r = ctx->r;
@@ -1238,9 +1349,13 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
     g.obs_part = ctx->obs_part;
     g.N = ctx->p.n_r;
     g.Nl = ctx->p.n_l;
-    g.Nlp = ctx->Nlp;
+    g.Nlx = ctx->p.n_l;
     g.Nth = ctx->p.n_theta;
-    g.Nthp = ctx->Nthp;
+    g.h = ctx->ang.h;
+    g.hq = ctx->ang.hq;
+    g.sS = ctx->ang.sS;
+    g.sA = ctx->ang.sA;
+    g.nblk = ctx->ang.nblk;
     g.C = ctx->C;
     g.n_orb = ctx->p.n_orb;
     g.n_field = ctx->n_field;
@@ -1250,19 +1365,24 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
     return g;
 }
 
-static size_t ang_lds(ggb_ctx *ctx) {
-    return (size_t)(ctx->Nlp + ctx->Nthp) * 16 * (ctx->p.precision ? sizeof(float) : sizeof(double));
+template <int MODE, typename T>
+static void launch_angular_t(ggb_ctx *ctx, const AngArgs &g) {
+    const dim3 ga(ctx->ntiles, ctx->C);
+    const ang_geom &ag = ctx->ang;
+    if (ag.glb)
+        k_angular<MODE, T, true><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
+    else
+        k_angular<MODE, T, false><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
 }
 
 template <int MODE>
 static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
-    const dim3 ga(ctx->ntiles, ctx->C);
     AngArgs g = ang_args(ctx, step);
     g.cur = cur;
     if (ctx->p.precision)
-        k_angular<MODE, float><<<ga, WG, ang_lds(ctx), ctx->stream>>>(g);
+        launch_angular_t<MODE, float>(ctx, g);
     else
-        k_angular<MODE, double><<<ga, WG, ang_lds(ctx), ctx->stream>>>(g);
+        launch_angular_t<MODE, double>(ctx, g);
 }
 
 // potential rebuild from the |Psi|^2 tiles already in ctx->dens
