@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libggbtd.so")
 
-GGB_ABI_VERSION = 1
+GGB_ABI_VERSION = 2
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "call out of order",
           -4: "device allocation failed"}
 
@@ -43,7 +43,8 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 class Static(ctypes.Structure):
     _fields_ = [("r", _dp), ("wr", _dp), ("v_gs", _dp), ("mask", _dp), ("Q", _dp),
                 ("lam", _dp), ("nkeep", _ip), ("B", _dp), ("PL", _dp), ("chi", _dp),
-                ("omega", _dp), ("m", _ip), ("occ", _dp), ("rho_frozen", _dp)]
+                ("omega", _dp), ("m", _ip), ("occ", _dp), ("rho_frozen", _dp),
+                ("phi", _dp), ("l0", _ip), ("eps", _dp)]
 
 
 class GGBError(RuntimeError):

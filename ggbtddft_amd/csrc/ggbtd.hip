@@ -58,15 +58,23 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 // Matrix-core product D += A B of one 16x16x4 tile in the compute type T
 // (FP64: v_mfma_f64_16x16x4f64, FP32: v_mfma_f32_16x16x4f32; same lane layout).
+// mixed: FP32 products on an FP64 state that is never rounded to FP32.  The
+// matrix-core products only see small quantities: the potential half step is
+// applied as psi += B^T ((f - 1) B psi), and the kinetic propagator acts on
+// the remainder psi - alpha phi of each orbital after its ground-state
+// component alpha phi (an eigenvector of exp(-i H_l dt), advanced exactly in
+// FP64 as alpha e^{-i eps dt} phi).  The operators are those of the FP64 form.
 template <typename T> struct mfma_t;
 template <> struct mfma_t<double> {
     typedef d4_t v4;
+    static constexpr bool mixed = false;
     static __device__ __forceinline__ v4 op(double a, double b, v4 c) { return mfma64(a, b, c); }
     // accumulator item r of lane L holds D[row(L / 16, r)][L % 16]
     static __device__ __forceinline__ int row(int kr, int r) { return 4 * r + kr; }
 };
 template <> struct mfma_t<float> {
     typedef f4_t v4;
+    static constexpr bool mixed = true;
     static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -77,6 +85,73 @@ static __device__ __forceinline__ double c_lm(int l, int m) {
     // <l m|cos theta|l+1 m> (internal.py:80-84, angularCoefficients.cuh d_c)
     double dl = (double)l, dm = (double)m;
     return sqrt(((dl + 1.0) * (dl + 1.0) - dm * dm) / ((2.0 * dl + 1.0) * (2.0 * dl + 3.0)));
+}
+
+// component p of e^{-i lam dt} (v_re + i v_im), from this lane's v and its
+// partner's vo
+template <typename T>
+static __device__ __forceinline__ T kin_factor(double v, double vo, int p, double lam, double dt) {
+    const double re = p ? vo : v, im = p ? v : vo;
+    double sn, cs;
+    sincos(lam * dt, &sn, &cs);
+    return (T)(p ? (cs * im - sn * re) : (cs * re + sn * im));
+}
+
+// Ground-state reference of the mixed form: for column c (orbital k = c mod
+// n_orb, block l0[k]) alpha = <phi_k|psi_l0[c]> and beta = alpha e^{-i eps_k dt},
+// FP64, fixed-order reduction.  ab[c] = (alpha_re, alpha_im, beta_re, beta_im).
+struct KinRef {
+    const double *phi;   // [n_orb][N]
+    const int32_t *l0;   // [n_orb]
+    const double4 *ab;   // [C]
+};
+__global__ __launch_bounds__(WG) void k_kin_ref(const double *__restrict__ psi,
+                                                const double *__restrict__ phi,
+                                                const int32_t *__restrict__ l0,
+                                                const double *__restrict__ eps, double4 *ab,
+                                                int N, int C, int n_orb, double dt) {
+    __shared__ double red[2][WG / 64];
+    const int c = blockIdx.x, k = c % n_orb;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const double *ph = phi + (size_t)k * N;
+    const double2 *x = (const double2 *)psi + ((size_t)l0[k] * C + c) * N;
+    double sr = 0.0, si = 0.0;
+    for (int i = t; i < N; i += WG) {
+        const double2 v = x[i];
+        sr += ph[i] * v.x;
+        si += ph[i] * v.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_down(sr, off);
+        si += __shfl_down(si, off);
+    }
+    if (lane == 0) {
+        red[0][wave] = sr;
+        red[1][wave] = si;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double ar = 0.0, ai = 0.0;
+        for (int w = 0; w < WG / 64; ++w) {
+            ar += red[0][w];
+            ai += red[1][w];
+        }
+        double sn, cs;
+        sincos(eps[k] * dt, &sn, &cs);
+        ab[c] = make_double4(ar, ai, cs * ar + sn * ai, cs * ai - sn * ar);
+    }
+}
+
+// reference component of lane column (c, p) in block l: coefficient (alpha or
+// beta, component p) and phi row, or a null row
+static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, int l, int c, int p,
+                                                            int N, int n_orb, bool beta,
+                                                            double *coef) {
+    const int k = c % n_orb;
+    if (!kr.phi || kr.l0[k] != l) return nullptr;
+    const double4 a = kr.ab[c];
+    *coef = beta ? (p ? a.w : a.z) : (p ? a.y : a.x);
+    return kr.phi + (size_t)k * N;
 }
 
 // ---------------------------------------------------------------------------
@@ -93,9 +168,9 @@ template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 const double *__restrict__ lam,
                                                 const int32_t *__restrict__ nkeep,
-                                                const double *__restrict__ psi,
+                                                const double *__restrict__ psi, KinRef ref,
                                                 T *__restrict__ Y, int N, int nk_max, int C,
-                                                double dt) {
+                                                int n_orb, double dt) {
     typedef typename mfma_t<T>::v4 v4;
     __shared__ double red[3][64][4];
     const int l = blockIdx.z;
@@ -111,6 +186,8 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
     const double *Xl = psi + ((size_t)l * C + (colok ? c : 0)) * N * 2 + p;
     const int kq = (((N + 3) / 4 + 3) / 4) * 4;  // quarter of K, multiple of 4
     const int kb = wave * kq, ke = min(N, kb + kq);
+    double rc = 0.0;
+    const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, false, &rc) : nullptr;
     v4 acc = {0, 0, 0, 0};
     for (int k0 = kb; k0 < ke; k0 += 4 * KU) {
         T av[KU], bv[KU];
@@ -119,7 +196,9 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
             const int i = k0 + 4 * u + kr;
             const bool ok = i < ke;
             av[u] = ok ? Ql[(size_t)i * nk_max] : (T)0;
-            bv[u] = (ok && colok) ? (T)Xl[(size_t)i * 2] : (T)0;
+            double x = (ok && colok) ? Xl[(size_t)i * 2] : 0.0;
+            if (rrow && ok) x -= rc * rrow[i];
+            bv[u] = (T)x;
         }
 #pragma unroll
         for (int u = 0; u < KU; ++u) acc = mfma_t<T>::op(av[u], bv[u], acc);
@@ -140,10 +219,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
         const int a = a0 + mfma_t<T>::row(kr, r);
         const double v = v4d[r];
         const double vo = __shfl_xor(v, 1);
-        const double re = p ? vo : v, im = p ? v : vo;
-        double sn, cs;
-        sincos(laml[a < nk ? a : 0] * dt, &sn, &cs);
-        o[r] = (T)(p ? (cs * im - sn * re) : (cs * re + sn * im));
+        o[r] = kin_factor<T>(v, vo, p, laml[a < nk ? a : 0], dt);
     }
     if (colok) {
         T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max + a0;
@@ -167,9 +243,9 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
                                                 const int32_t *__restrict__ nkeep,
-                                                const T *__restrict__ Y,
+                                                const T *__restrict__ Y, KinRef ref,
                                                 double *__restrict__ psi, int N, int nk_max,
-                                                int C) {
+                                                int C, int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
     __shared__ double red[3][64][4];
     const int l = blockIdx.z;
@@ -210,11 +286,16 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
     __syncthreads();
     if (wave > 0) return;
     double *Xl = psi + (size_t)l * C * N * 2;
+    double rc = 0.0;
+    const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, true, &rc) : nullptr;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = i0 + mfma_t<T>::row(kr, r);
-        const double v = (((double)acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
-        if (colok && i < N) Xl[((size_t)c * N + i) * 2 + p] = v;
+        double v = (((double)acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+        if (colok && i < N) {
+            if (rrow) v += rc * rrow[i];
+            Xl[((size_t)c * N + i) * 2 + p] = v;
+        }
     }
 }
 
@@ -233,9 +314,9 @@ template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
                                                       const double *__restrict__ lam,
                                                       const int32_t *__restrict__ nkeep,
-                                                      const double *__restrict__ psi,
+                                                      const double *__restrict__ psi, KinRef ref,
                                                       T *__restrict__ Y, int N, int nk_max,
-                                                      int C, double dt) {
+                                                      int C, int n_orb, double dt) {
     typedef typename mfma_t<T>::v4 v4;
     __shared__ T As[2][TK][TLD];  // [k][m]
     __shared__ T Bs[2][TK][TLD];  // [k][n]
@@ -254,6 +335,9 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
     const int bc = t >> 3, bk = (t & 7) * 2;           // B: complex column, 2 consecutive k
     const int cg = (n0 >> 1) + bc;
     const bool cok = cg < C;
+    double rcr = 0.0, rci = 0.0;
+    const double *rrow = (mfma_t<T>::mixed && cok) ? kin_ref_row(ref, l, cg, 0, N, n_orb, false, &rcr) : nullptr;
+    if (rrow) kin_ref_row(ref, l, cg, 1, N, n_orb, false, &rci);
     T ra[4], rb[4];
     auto load = [&](int k0) {
         const int i = k0 + ak;
@@ -264,8 +348,14 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
         for (int u = 0; u < 2; ++u) {
             const int ii = k0 + bk + u;
             const bool ok = cok && ii < N;
-            rb[2 * u] = ok ? (T)Xl[((size_t)cg * N + ii) * 2] : (T)0;
-            rb[2 * u + 1] = ok ? (T)Xl[((size_t)cg * N + ii) * 2 + 1] : (T)0;
+            double xr = ok ? Xl[((size_t)cg * N + ii) * 2] : 0.0;
+            double xi = ok ? Xl[((size_t)cg * N + ii) * 2 + 1] : 0.0;
+            if (rrow && ok) {
+                xr -= rcr * rrow[ii];
+                xi -= rci * rrow[ii];
+            }
+            rb[2 * u] = (T)xr;
+            rb[2 * u + 1] = (T)xi;
         }
     };
     auto stash = [&](int buf) {
@@ -317,10 +407,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
                 const int a = ab + mfma_t<T>::row(kr, r);
                 const double v = (double)acc[x][y][r];
                 const double vo = __shfl_xor(v, 1);
-                const double re = p ? vo : v, im = p ? v : vo;
-                double sn, cs;
-                sincos(laml[a < nk ? a : 0] * dt, &sn, &cs);
-                o[r] = (T)(p ? (cs * im - sn * re) : (cs * re + sn * im));
+                o[r] = kin_factor<T>(v, vo, p, laml[a < nk ? a : 0], dt);
             }
             if (c < C) {
                 T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max;
@@ -337,9 +424,9 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
                                                       const int32_t *__restrict__ nkeep,
-                                                      const T *__restrict__ Y,
+                                                      const T *__restrict__ Y, KinRef ref,
                                                       double *__restrict__ psi, int N, int nk_max,
-                                                      int C) {
+                                                      int C, int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
     __shared__ T As[2][TK][TLD];  // [k = a][m = i]
     __shared__ T Bs[2][TK][TLD];  // [k = a][n = col]
@@ -411,10 +498,16 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
             const int col = n0 + wn + 16 * y + lc;
             const int c = col >> 1, p = col & 1;
             if (c >= C) continue;
+            double rc = 0.0;
+            const double *rrow = mfma_t<T>::mixed ? kin_ref_row(ref, l, c, p, N, n_orb, true, &rc) : nullptr;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = i0 + wm + 16 * x + mfma_t<T>::row(kr, r);
-                if (i < N) Xl[((size_t)c * N + i) * 2 + p] = (double)acc[x][y][r];
+                if (i < N) {
+                    double v = (double)acc[x][y][r];
+                    if (rrow) v += rc * rrow[i];
+                    Xl[((size_t)c * N + i) * 2 + p] = v;
+                }
             }
         }
     }
@@ -434,9 +527,12 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
 // of the dense N_theta x N_l transform, and the two half tables of one m
 // (hq x (sS + sA) values) are staged in LDS once per work-group (GLB = true
 // reads them from global memory when they do not fit).
-// LDS: tables | X[Nl][16] | S[hq][16] | D[hq][16] in the compute type T (real
-// column n = 2 * i_local + re/im); the state in HBM, the phase factors and
-// the observable sums stay FP64.
+// LDS: tables (T) | X[Nl][16] (FP64) | S[hq][16] | D[hq][16] (T), real column
+// n = 2 * i_local + re/im.  The state tile X, the laser rotations, the phase
+// factors and the observable sums are FP64 in both forms; the mixed form
+// applies the potential half step as X += B^T ((f - 1) B X).  The dipole-type
+// sums use sum_j chi_j |Psi_j|^2 = sum_{j<h} chi_j 4 Re(E_j^* O_j), so that the
+// large even part of a core orbital never enters them through a difference.
 // mode 0: Vh, Le, Lo          (first half of the step)
 // mode 1: Lo, Le, Vh, store, then |B psi|^2 + observable partials
 // mode 2: |B psi|^2 + observable partials only (initial potential)
@@ -460,8 +556,7 @@ struct AngArgs {
     double dt;
 };
 
-template <typename T>
-static __device__ void ang_laser(T *X, int parity, int m, int Nl, const double *a_i) {
+static __device__ void ang_laser(double *X, int parity, int m, int Nl, const double *a_i) {
     // pairs (l, l+1), l = m + parity + 2 q, l + 1 < Nl; thread -> (q, i_local)
     const int t = threadIdx.x;
     const int il = t & 7;
@@ -469,12 +564,11 @@ static __device__ void ang_laser(T *X, int parity, int m, int Nl, const double *
         const int l = m + parity + 2 * q;
         if (l + 1 >= Nl) break;
         const double th = a_i[il] * c_lm(l, m);
-        double snd, csd;
-        sincos(th, &snd, &csd);
-        const T sn = (T)snd, cs = (T)csd;
-        T *x0 = X + l * 16 + 2 * il;
-        T *x1 = X + (l + 1) * 16 + 2 * il;
-        const T r0 = x0[0], i0 = x0[1], r1 = x1[0], i1 = x1[1];
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        double *x0 = X + l * 16 + 2 * il;
+        double *x1 = X + (l + 1) * 16 + 2 * il;
+        const double r0 = x0[0], i0 = x0[1], r1 = x1[0], i1 = x1[1];
         // [x0', x1'] = [[cs, -i sn], [-i sn, cs]] [x0, x1]
         x0[0] = cs * r0 + sn * i1;
         x0[1] = cs * i0 - sn * r1;
@@ -497,7 +591,7 @@ struct ang_par {
 // forward product of theta tile `tile` (rows j = 16 tile + 0..15 of the first
 // half): e = Bs X_s, o = Ba X_a; two independent accumulator chains.
 template <typename T>
-static __device__ void ang_fwd(const T *Bs, const T *Ba, const T *X, const AngArgs &g,
+static __device__ void ang_fwd(const T *Bs, const T *Ba, const double *X, const AngArgs &g,
                                const ang_par &pr, int m, int tile, typename mfma_t<T>::v4 &e,
                                typename mfma_t<T>::v4 &o) {
     typedef typename mfma_t<T>::v4 v4;
@@ -513,13 +607,13 @@ static __device__ void ang_fwd(const T *Bs, const T *Ba, const T *X, const AngAr
         if (k0 < pr.ks) {
             const int l = m + 2 * k;
             const T a = jok ? As[k] : (T)0;
-            const T b = l < g.Nl ? X[l * 16 + lc] : (T)0;
+            const T b = l < g.Nl ? (T)X[l * 16 + lc] : (T)0;
             ae = mfma_t<T>::op(a, b, ae);
         }
         if (k0 < pr.ka) {
             const int l = m + 1 + 2 * k;
             const T a = jok ? Aa[k] : (T)0;
-            const T b = l < g.Nl ? X[l * 16 + lc] : (T)0;
+            const T b = l < g.Nl ? (T)X[l * 16 + lc] : (T)0;
             ao = mfma_t<T>::op(a, b, ao);
         }
     }
@@ -530,7 +624,7 @@ static __device__ void ang_fwd(const T *Bs, const T *Ba, const T *X, const AngAr
 // back product of output tile `item` (s tiles, then a tiles) into X; two
 // accumulator chains over alternating groups of 4 nodes, added at the end.
 template <typename T>
-static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D, T *X,
+static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D, double *X,
                                const AngArgs &g, const ang_par &pr, int m, int item) {
     typedef typename mfma_t<T>::v4 v4;
     const int lane = threadIdx.x & 63, kr = lane >> 4, lc = lane & 15;
@@ -563,7 +657,11 @@ static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int ko = tile * 16 + mfma_t<T>::row(kr, r);
-        if (ko < nn) X[(m + off + 2 * ko) * 16 + lc] = a0[r] + a1[r];
+        if (ko < nn) {
+            double &x = X[(m + off + 2 * ko) * 16 + lc];
+            const double v = (double)(a0[r] + a1[r]);
+            x = mfma_t<T>::mixed ? x + v : v;
+        }
     }
 }
 
@@ -588,9 +686,9 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
 
     const T *Bsrc = reinterpret_cast<const T *>(g.Bh) + (size_t)m * g.nblk;
     T *Tl = reinterpret_cast<T *>(lds);
-    T *X = GLB ? Tl : Tl + g.nblk;  // [Nlx][16]
-    T *S = X + g.Nlx * 16;          // [hq][16]
-    T *D = S + g.hq * 16;           // [hq][16]
+    double *X = reinterpret_cast<double *>(GLB ? Tl : Tl + g.nblk);  // [Nlx][16]
+    T *S = reinterpret_cast<T *>(X + g.Nlx * 16);                    // [hq][16]
+    T *D = S + g.hq * 16;                                            // [hq][16]
     const T *Bs = GLB ? Bsrc : Tl;
     const T *Ba = Bs + (size_t)g.hq * g.sS;
 
@@ -607,8 +705,8 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         const int i = i0 + il;
         double2 v = make_double2(0.0, 0.0);
         if (l < g.Nl && i < g.N) v = ((const double2 *)g.psi)[((size_t)l * g.C + c) * g.N + i];
-        X[l * 16 + 2 * il] = (T)v.x;
-        X[l * 16 + 2 * il + 1] = (T)v.y;
+        X[l * 16 + 2 * il] = v.x;
+        X[l * 16 + 2 * il + 1] = v.y;
     }
     if (MODE != 2 && t < RT) {
         const int i = i0 + t;
@@ -619,9 +717,9 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     __syncthreads();
 
     if (MODE == 1) {
-        ang_laser<T>(X, 1, m, g.Nl, a_i);
+        ang_laser(X, 1, m, g.Nl, a_i);
         __syncthreads();
-        ang_laser<T>(X, 0, m, g.Nl, a_i);
+        ang_laser(X, 0, m, g.Nl, a_i);
         __syncthreads();
     }
 
@@ -641,11 +739,15 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                 const T pao = __shfl_xor(pa, 1), pbo = __shfl_xor(pb, 1);
                 T sv = (T)0, dv = (T)0;
                 if (j < g.h && iok) {
+                    // mixed form: (f - 1), formed in FP64 before the rounding
+                    const double one = mfma_t<T>::mixed ? 1.0 : 0.0;
                     const double2 fa = ph[j], fb = ph[jm];
+                    const T fax = (T)(fa.x - one), fay = (T)fa.y;
+                    const T fbx = (T)(fb.x - one), fby = (T)fb.y;
                     const T ra = p ? pao : pa, ia = p ? pa : pao;
                     const T rb = p ? pbo : pb, ib = p ? pb : pbo;
-                    const T qa = p ? (ra * (T)fa.y + ia * (T)fa.x) : (ra * (T)fa.x - ia * (T)fa.y);
-                    const T qb = p ? (rb * (T)fb.y + ib * (T)fb.x) : (rb * (T)fb.x - ib * (T)fb.y);
+                    const T qa = p ? (ra * fay + ia * fax) : (ra * fax - ia * fay);
+                    const T qb = p ? (rb * fby + ib * fbx) : (rb * fbx - ib * fby);
                     sv = (jm == j) ? qa : qa + qb;
                     dv = (jm == j) ? qa : qa - qb;
                 }
@@ -661,9 +763,9 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     }
 
     if (MODE == 0) {
-        ang_laser<T>(X, 0, m, g.Nl, a_i);
+        ang_laser(X, 0, m, g.Nl, a_i);
         __syncthreads();
-        ang_laser<T>(X, 1, m, g.Nl, a_i);
+        ang_laser(X, 1, m, g.Nl, a_i);
         __syncthreads();
     }
 
@@ -689,21 +791,21 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         for (int r = 0; r < 4; ++r) {
             const int j = wave * 16 + mfma_t<T>::row(kr, r);
             const int jm = g.Nth - 1 - j;
-            const double va = (double)(e[r] + o[r]), vb = (double)(e[r] - o[r]);
+            const double ed = (double)e[r], od = (double)o[r];
+            const double va = ed + od, vb = ed - od;
             const double s2a = va * va + __shfl_xor(va * va, 1);
             const double s2b = vb * vb + __shfl_xor(vb * vb, 1);
+            const double eo = 4.0 * (ed * od + __shfl_xor(ed * od, 1));  // |Psi_j|^2 - |Psi_j'|^2
             if (p == 0 && iok && j < g.h) {
-                const double ca = g.chi[j];
                 dens[j] = s2a;
-                sd += ri * ca * s2a;
-                sa += ca * s2a / (ri * ri);
-                sn += s2a;
                 if (jm != j) {
-                    const double cb = g.chi[jm];
+                    const double ca = g.chi[j];
                     dens[jm] = s2b;
-                    sd += ri * cb * s2b;
-                    sa += cb * s2b / (ri * ri);
-                    sn += s2b;
+                    sd += ri * ca * eo;
+                    sa += ca * eo / (ri * ri);
+                    sn += s2a + s2b;
+                } else {
+                    sn += s2a;  // middle node, chi = 0
                 }
             }
         }
@@ -784,57 +886,88 @@ static __device__ T lda_vxc(T rho) {
     return vx + eps - rs / (T)3 * deps;
 }
 
-// Density, LDA v_xc (in T) and multipole moments for one instance and RTM
-// radial nodes: rho_u and v_xc pointwise, then q_L(i) = sum_j P_L(chi_j)
-// rho_u(i, j) as an FP64 matrix-core product (M = L, N = radial node, K = j).
-#define RTM 16
+// Density and LDA v_xc (in T) pointwise on the (r_i, theta_j) grid:
+// rho_u(i, j) = sum_k occ_k |Psi_k(i, j)|^2 (+ the frozen core), kept for the
+// multipole moments, and v_xc of the density rho = rho_u / (2 pi omega_j wr_i r_i^2).
 template <typename T>
-__global__ __launch_bounds__(WG) void k_density(const double *__restrict__ dens,
-                                                const double *__restrict__ occ,
-                                                const double *__restrict__ rho_frozen,
-                                                const double *__restrict__ r,
-                                                const double *__restrict__ wr,
-                                                const double *__restrict__ omega,
-                                                const double *__restrict__ PLT,
-                                                double *__restrict__ vxc, double *__restrict__ q,
-                                                int N, int Nth, int n_orb, int n_har) {
-    extern __shared__ double rho_s[];  // [Nthp][RTM], zero padded
-    const int i0 = blockIdx.x * RTM, inst = blockIdx.y;
-    const int t = threadIdx.x;
-    const int Nthp = (Nth + 3) & ~3;
-    for (int idx = t; idx < Nthp * RTM; idx += WG) {
-        const int j = idx / RTM, il = idx % RTM;
-        const int i = i0 + il;
-        double ru = 0.0;
-        if (i < N && j < Nth) {
-            for (int k = 0; k < n_orb; ++k)
-                ru += occ[k] * dens[((size_t)(inst * n_orb + k) * N + i) * Nth + j];
-            if (rho_frozen) ru += 0.5 * rho_frozen[i] * omega[j];
-            const double rho = ru / (2.0 * M_PI * omega[j] * wr[i] * r[i] * r[i]);
-            vxc[((size_t)inst * N + i) * Nth + j] = (double)lda_vxc<T>((T)rho);
-        }
-        rho_s[idx] = ru;
-    }
-    __syncthreads();
-    const int wave = t >> 6, lane = t & 63, kr = lane >> 4, lc = lane & 15;
-    const int mt = (n_har + 15) / 16;
-    for (int tile = wave; tile < mt; tile += 4) {
-        const int L = tile * 16 + lc;
-        const double *Arow = PLT + (size_t)(L < n_har ? L : 0) * Nth;
-        d4_t acc = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < Nthp; k0 += 4) {
-            const int j = k0 + kr;
-            const double a = (L < n_har && j < Nth) ? Arow[j] : 0.0;
-            acc = mfma64(a, rho_s[j * RTM + lc], acc);
+__global__ __launch_bounds__(WG) void k_rho(const double *__restrict__ dens,
+                                            const double *__restrict__ occ,
+                                            const double *__restrict__ rho_frozen,
+                                            const double *__restrict__ r,
+                                            const double *__restrict__ wr,
+                                            const double *__restrict__ omega,
+                                            double *__restrict__ rho_u, double *__restrict__ vxc,
+                                            int N, int Nth, int n_orb) {
+    const int inst = blockIdx.y;
+    const size_t idx = (size_t)blockIdx.x * WG + threadIdx.x;
+    if (idx >= (size_t)N * Nth) return;
+    const int i = (int)(idx / Nth), j = (int)(idx % Nth);
+    double ru = 0.0;
+    for (int k = 0; k < n_orb; ++k) ru += occ[k] * dens[(size_t)(inst * n_orb + k) * N * Nth + idx];
+    if (rho_frozen) ru += 0.5 * rho_frozen[i] * omega[j];
+    const double rho = ru / (2.0 * M_PI * omega[j] * wr[i] * r[i] * r[i]);
+    const size_t o = (size_t)inst * N * Nth + idx;
+    rho_u[o] = ru;
+    vxc[o] = (double)lda_vxc<T>((T)rho);
+}
+
+// One 16 x 16 tile of D = A B in FP64 on the matrix cores per work-group
+// (tile (blockIdx.y, blockIdx.x), instance blockIdx.z), the K range split
+// over the four waves and the partial tiles added in fixed order.
+// A[m][k] = A[z saz + m sam + k sak], B[k][n] = B[z sbz + k sbk + n sbn], zero
+// outside M x K and K x N.  Wave 0 hands each element to epi(z, m, n, value).
+struct GemmOperands {
+    const double *A, *B;
+    long sam, sak, saz, sbk, sbn, sbz;
+    int M, N, K;
+};
+template <class Epi>
+__global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
+    __shared__ double red[3][64][4];
+    const int z = blockIdx.z;
+    const int m0 = blockIdx.y * 16, n0 = blockIdx.x * 16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int m = m0 + lc, n = n0 + lc;
+    const bool mok = m < g.M, nok = n < g.N;
+    const double *Ar = g.A + z * g.saz + (long)(mok ? m : 0) * g.sam;
+    const double *Bc = g.B + z * g.sbz + (long)(nok ? n : 0) * g.sbn;
+    const int kq = (((g.K + 3) / 4 + 3) / 4) * 4;  // quarter of K, multiple of 4
+    const int kb = wave * kq, ke = min(g.K, kb + kq);
+    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + 4 * u + kr;
+            const bool ok = k < ke;
+            av[u] = (ok && mok) ? Ar[(long)k * g.sak] : 0.0;
+            bv[u] = (ok && nok) ? Bc[(long)k * g.sbk] : 0.0;
         }
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            const int Lo = tile * 16 + mfma_t<double>::row(kr, rr);
-            const int i = i0 + lc;
-            if (Lo < n_har && i < N) q[((size_t)inst * n_har + Lo) * N + i] = acc[rr];
-        }
+        for (int u = 0; u < 4; ++u) acc = mfma64(av[u], bv[u], acc);
+    }
+    if (wave > 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave - 1][lane][r] = acc[r];
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int mo = m0 + mfma_t<double>::row(kr, r);
+        const double v = ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+        if (mo < g.M && nok) epi(z, mo, n, v);
     }
 }
+
+// multipole moments q_L(i) = sum_j P_L(chi_j) rho_u(i, j): M = L, N = i, K = j
+struct QmomEpi {
+    double *q;
+    int N, n_har;
+    __device__ void operator()(int z, int L, int i, double v) const {
+        q[((size_t)z * n_har + L) * N + i] = v;
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Hartree multipole V_L(r_a) = sum_b q_L(b) r_<^L / r_>^{L+1} by two affine
@@ -934,52 +1067,23 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
     }
 }
 
-// dV = sum_L P_L(chi_j) V_L(r_i) + v_xc - v_gs and phase = mask e^{-i dV dt/2}
-// for one instance and RTM radial nodes; the Legendre sum is an FP64
-// matrix-core product (M = j, N = radial node, K = L).
-__global__ __launch_bounds__(WG) void k_dv(const double *__restrict__ V,
-                                           const double *__restrict__ vxc,
-                                           const double *__restrict__ PL,
-                                           const double *__restrict__ v_gs,
-                                           const double *__restrict__ mask,
-                                           double *__restrict__ dv, double2 *__restrict__ phase,
-                                           int N, int Nth, int n_har, double dt, int interacting) {
-    extern __shared__ double Vt[];  // [n_harp][RTM], zero padded
-    const int i0 = blockIdx.x * RTM, inst = blockIdx.y;
-    const int t = threadIdx.x;
-    const int nhp = (n_har + 3) & ~3;
-    for (int idx = t; idx < nhp * RTM; idx += WG) {
-        const int L = idx / RTM, il = idx % RTM;
-        const int i = i0 + il;
-        Vt[idx] = (interacting && L < n_har && i < N) ? V[((size_t)inst * n_har + L) * N + i] : 0.0;
+// dV = sum_L V_L(r_i) P_L(chi_j) + v_xc - v_gs and phase = mask e^{-i dV dt/2}:
+// the Legendre sum as a tile product (M = i, N = j, K = L), this epilogue.
+struct DvEpi {
+    const double *vxc, *v_gs, *mask;
+    double *dv;
+    double2 *phase;
+    int N, Nth, interacting;
+    double dt;
+    __device__ void operator()(int z, int i, int j, double v) const {
+        const size_t o = ((size_t)z * N + i) * Nth + j;
+        const double sv = interacting ? v + vxc[o] - v_gs[i] : 0.0;
+        dv[o] = sv;
+        double sn, cs;
+        sincos(0.5 * dt * sv, &sn, &cs);
+        phase[o] = make_double2(mask[i] * cs, -mask[i] * sn);
     }
-    __syncthreads();
-    const int wave = t >> 6, lane = t & 63, kr = lane >> 4, lc = lane & 15;
-    const int mt = (Nth + 15) / 16;
-    const int i = i0 + lc;
-    for (int tile = wave; tile < mt; tile += 4) {
-        const int j = tile * 16 + lc;
-        const double *Arow = PL + (size_t)(j < Nth ? j : 0) * n_har;
-        d4_t acc = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < nhp; k0 += 4) {
-            const int L = k0 + kr;
-            const double a = (j < Nth && L < n_har) ? Arow[L] : 0.0;
-            acc = mfma64(a, Vt[L * RTM + lc], acc);
-        }
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            const int jo = tile * 16 + mfma_t<double>::row(kr, rr);
-            if (jo >= Nth || i >= N) continue;
-            const size_t o = ((size_t)inst * N + i) * Nth + jo;
-            double sv = 0.0;
-            if (interacting) sv = acc[rr] + vxc[o] - v_gs[i];
-            dv[o] = sv;
-            double sn, cs;
-            sincos(0.5 * dt * sv, &sn, &cs);
-            phase[o] = make_double2(mask[i] * cs, -mask[i] * sn);
-        }
-    }
-}
+};
 
 // plain streaming copy of known size (calibrates the HBM byte counters)
 __global__ __launch_bounds__(WG) void k_copy(const double2 *__restrict__ src,
@@ -1024,7 +1128,7 @@ static ang_geom ang_geometry(const ggb_problem *p) {
     a.nblk = ((a.hq * (a.sS + a.sA) + per16 - 1) / per16) * per16;
     const int mt = (a.h + 15) / 16, nb = (ns0 + 15) / 16 + (na0 + 15) / 16;
     a.waves = mt > nb ? mt : nb;
-    const size_t tiles = (size_t)(p->n_l * 16 + 2 * a.hq * 16) * es;
+    const size_t tiles = (size_t)p->n_l * 16 * sizeof(double) + (size_t)2 * a.hq * 16 * es;
     const size_t stat = (RT + 3 * ANG_MAXW) * sizeof(double);
     a.glb = tiles + (size_t)a.nblk * es + stat > ANG_LDS_MAX;
     a.lds = a.glb ? tiles : tiles + (size_t)a.nblk * es;
@@ -1043,8 +1147,13 @@ struct ggb_ctx {
     // device buffers
     double *psi, *Y, *Q, *lam, *B, *PL, *PLT, *chi, *omega, *r, *lr, *wr, *v_gs, *mask, *occ, *rho_frozen;
     int32_t *nkeep, *mcol;
-    double *dens, *vxc, *q, *V, *dv, *obs_part, *obs, *E;
+    double *dens, *vxc, *rho_u, *q, *V, *dv, *obs_part, *obs, *E;
     double2 *phase;
+    // mixed form: ground-state reference of each orbital and its overlaps
+    bool have_ref;
+    double *phi_ref, *eps;
+    int32_t *l0;
+    double4 *ab;
     size_t obs_cap;
     // optional hipGraph of graph_steps steps (ggb_set_graph)
     int graph_steps;
@@ -1101,8 +1210,9 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->lam, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
                     ctx->r, ctx->lr, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
-                    ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->q, ctx->V, ctx->dv,
-                    ctx->obs_part, ctx->obs, ctx->E, ctx->phase};
+                    ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
+                    ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
+                    ctx->l0, ctx->ab};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
@@ -1124,7 +1234,7 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
     d += 6 * N + p->n_orb;               // r, lr, wr, v_gs, mask, rho_frozen, occ
     d += C * N * Nth;                    // dens
-    d += 2 * nI * N * Nth;               // vxc, dv
+    d += 3 * nI * N * Nth;               // vxc, rho_u, dv
     d += 2 * nI * N * Nth;               // phase (double2)
     d += 2 * nI * nh * N;                // q, V
     d += C * (size_t)ntiles * 3;         // obs_part
@@ -1170,8 +1280,12 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->psi = ctx->Y = ctx->Q = ctx->lam = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
     ctx->r = ctx->lr = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
     ctx->nkeep = ctx->mcol = nullptr;
-    ctx->dens = ctx->vxc = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
+    ctx->dens = ctx->vxc = ctx->rho_u = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
     ctx->phase = nullptr;
+    ctx->have_ref = false;
+    ctx->phi_ref = ctx->eps = nullptr;
+    ctx->l0 = nullptr;
+    ctx->ab = nullptr;
     ctx->stream = nullptr;
     ctx->timing = 0;
     ctx->graph_steps = 0;
@@ -1215,11 +1329,18 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ok &= dalloc(&ctx->mcol, (size_t)p->n_orb) == hipSuccess;
     ok &= dalloc(&ctx->dens, C * N * Nth) == hipSuccess;
     ok &= dalloc(&ctx->vxc, nI * N * Nth) == hipSuccess;
+    ok &= dalloc(&ctx->rho_u, nI * N * Nth) == hipSuccess;
     ok &= dalloc(&ctx->q, nI * (size_t)p->n_har * N) == hipSuccess;
     ok &= dalloc(&ctx->V, nI * (size_t)p->n_har * N) == hipSuccess;
     ok &= dalloc(&ctx->dv, nI * N * Nth) == hipSuccess;
     ok &= dalloc(&ctx->phase, nI * N * Nth) == hipSuccess;
     ok &= dalloc(&ctx->obs_part, C * (size_t)ctx->ntiles * 3) == hipSuccess;
+    if (p->precision) {
+        ok &= dalloc(&ctx->phi_ref, (size_t)p->n_orb * N) == hipSuccess;
+        ok &= dalloc(&ctx->eps, (size_t)p->n_orb) == hipSuccess;
+        ok &= dalloc(&ctx->l0, (size_t)p->n_orb) == hipSuccess;
+        ok &= dalloc(&ctx->ab, C) == hipSuccess;
+    }
     if (!ok) return bail("ggb_create: device allocation failed", GGB_E_NOMEM);
     // zero everything that kernels may read before it is written
     if (hipMemset(ctx->B, 0, nB * es) != hipSuccess || hipMemset(ctx->Q, 0, nQ * es) != hipSuccess ||
@@ -1321,6 +1442,16 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->chi, s->chi, Nth * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->omega, s->omega, Nth * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->mcol, s->m, p.n_orb * 4, hipMemcpyHostToDevice, st));
+    ctx->have_ref = false;
+    if (p.precision && s->phi && s->l0 && s->eps) {
+        for (int k = 0; k < p.n_orb; ++k)
+            if (s->l0[k] < s->m[k] || s->l0[k] >= p.n_l)
+                return fail(ctx, GGB_E_ARG, "ggb_upload_static: reference l0 out of range");
+        HIPCHK(ctx, hipMemcpy(ctx->phi_ref, s->phi, (size_t)p.n_orb * N * 8, hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(ctx->eps, s->eps, (size_t)p.n_orb * 8, hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(ctx->l0, s->l0, (size_t)p.n_orb * 4, hipMemcpyHostToDevice));
+        ctx->have_ref = true;
+    }
     HIPCHK(ctx, hipMemcpyAsync(ctx->occ, s->occ, p.n_orb * 8, hipMemcpyHostToDevice, st));
     if (s->rho_frozen) {
         if (!ctx->rho_frozen) HIPCHK(ctx, dalloc(&ctx->rho_frozen, N));
@@ -1389,24 +1520,45 @@ static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
 static int launch_potential(ggb_ctx *ctx) {
     const ggb_problem &p = ctx->p;
     hipStream_t st = ctx->stream;
-    const dim3 gt((p.n_r + RTM - 1) / RTM, p.n_inst);
-    const size_t lds_d = (size_t)((p.n_theta + 3) & ~3) * RTM * 8;
-    const size_t lds_v = (size_t)((p.n_har + 3) & ~3) * RTM * 8;
+    const int N = p.n_r, Nth = p.n_theta, nh = p.n_har;
     if (p.interacting) {
+        const dim3 gr((unsigned)(((size_t)N * Nth + WG - 1) / WG), p.n_inst);
         if (p.precision)
-            k_density<float><<<gt, WG, lds_d, st>>>(
-                ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PLT,
-                ctx->vxc, ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
+            k_rho<float><<<gr, WG, 0, st>>>(ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
+                                            ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb);
         else
-            k_density<double><<<gt, WG, lds_d, st>>>(
-                ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr, ctx->omega, ctx->PLT,
-                ctx->vxc, ctx->q, p.n_r, p.n_theta, p.n_orb, p.n_har);
-        k_hartree<<<dim3(p.n_har, p.n_inst), WG, 0, st>>>(ctx->q, ctx->r, ctx->lr, ctx->V, p.n_r,
-                                                           p.n_har);
+            k_rho<double><<<gr, WG, 0, st>>>(ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
+                                             ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb);
+        GemmOperands gq;  // q[L][i] = sum_j PLT[L][j] rho_u[i][j]
+        gq.A = ctx->PLT;
+        gq.sam = Nth;
+        gq.sak = 1;
+        gq.saz = 0;
+        gq.B = ctx->rho_u;
+        gq.sbk = 1;
+        gq.sbn = Nth;
+        gq.sbz = (long)N * Nth;
+        gq.M = nh;
+        gq.N = N;
+        gq.K = Nth;
+        QmomEpi eq{ctx->q, N, nh};
+        k_tile_gemm<<<dim3((N + 15) / 16, (nh + 15) / 16, p.n_inst), WG, 0, st>>>(gq, eq);
+        k_hartree<<<dim3(nh, p.n_inst), WG, 0, st>>>(ctx->q, ctx->r, ctx->lr, ctx->V, N, nh);
     }
-    k_dv<<<gt, WG, lds_v, st>>>(ctx->V, ctx->vxc, ctx->PL, ctx->v_gs, ctx->mask,
-                                                   ctx->dv, ctx->phase, p.n_r, p.n_theta, p.n_har,
-                                                   p.dt, p.interacting);
+    GemmOperands gv;  // V(i, theta_j) = sum_L V[L][i] PLT[L][j]
+    gv.A = ctx->V;
+    gv.sam = 1;
+    gv.sak = N;
+    gv.saz = (long)nh * N;
+    gv.B = ctx->PLT;
+    gv.sbk = Nth;
+    gv.sbn = 1;
+    gv.sbz = 0;
+    gv.M = N;
+    gv.N = Nth;
+    gv.K = p.interacting ? nh : 0;
+    DvEpi ev{ctx->vxc, ctx->v_gs, ctx->mask, ctx->dv, ctx->phase, N, Nth, p.interacting, p.dt};
+    k_tile_gemm<<<dim3((Nth + 15) / 16, (N + 15) / 16, p.n_inst), WG, 0, st>>>(gv, ev);
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
 }
@@ -1422,23 +1574,30 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
     const ggb_problem &p = ctx->p;
     const T *Q = reinterpret_cast<const T *>(ctx->Q);
     T *Y = reinterpret_cast<T *>(ctx->Y);
+    KinRef ref;
+    ref.phi = (mfma_t<T>::mixed && ctx->have_ref) ? ctx->phi_ref : nullptr;
+    ref.l0 = ctx->l0;
+    ref.ab = ctx->ab;
+    if (pass == 0 && ref.phi)
+        k_kin_ref<<<ctx->C, WG, 0, ctx->stream>>>(ctx->psi, ctx->phi_ref, ctx->l0, ctx->eps, ctx->ab,
+                                                  p.n_r, ctx->C, p.n_orb, p.dt);
     if (ctx->C > KIN_NARROW_MAX_C) {
         const int nct = (2 * ctx->C + TB - 1) / TB;
         if (pass == 0)
             k_kin_fwd_tiled<T><<<dim3((p.nk_max + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
-                Q, ctx->lam, ctx->nkeep, ctx->psi, Y, p.n_r, p.nk_max, ctx->C, p.dt);
+                Q, ctx->lam, ctx->nkeep, ctx->psi, ref, Y, p.n_r, p.nk_max, ctx->C, p.n_orb, p.dt);
         else
             k_kin_bwd_tiled<T><<<dim3((p.n_r + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
-                Q, ctx->nkeep, Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
+                Q, ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max, ctx->C, p.n_orb);
         return;
     }
     const int ncol = (2 * ctx->C + 15) / 16;
     if (pass == 0)
         k_kin_fwd<T><<<dim3(p.nk_max / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
-            Q, ctx->lam, ctx->nkeep, ctx->psi, Y, p.n_r, p.nk_max, ctx->C, p.dt);
+            Q, ctx->lam, ctx->nkeep, ctx->psi, ref, Y, p.n_r, p.nk_max, ctx->C, p.n_orb, p.dt);
     else
         k_kin_bwd<T><<<dim3((p.n_r + 15) / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
-            Q, ctx->nkeep, Y, ctx->psi, p.n_r, p.nk_max, ctx->C);
+            Q, ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max, ctx->C, p.n_orb);
 }
 
 static int launch_kinetic_pass(ggb_ctx *ctx, int pass) {

@@ -59,6 +59,10 @@ class Propagator:
         s.nkeep = _lib.iptr(a["nkeep"])
         s.m = _lib.iptr(a["m"])
         s.rho_frozen = None
+        if all(k in a for k in ("phi", "l0", "eps")):
+            s.phi = _lib.dptr(a["phi"])
+            s.l0 = _lib.iptr(a["l0"])
+            s.eps = _lib.dptr(a["eps"])
         self._check(self.lib.ggb_upload_static(self.ctx, ctypes.byref(s)), "ggb_upload_static")
         self.n_field = 0
         self.upload_orbitals()

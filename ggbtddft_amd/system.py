@@ -89,7 +89,29 @@ class AtomSystem:
                     nkeep=np.ascontiguousarray(self.nkeep), B=np.ascontiguousarray(a.B),
                     PL=np.ascontiguousarray(a.PL), chi=np.ascontiguousarray(a.x),
                     omega=np.ascontiguousarray(a.omega), m=np.ascontiguousarray(self.m),
-                    occ=np.ascontiguousarray(self.occ))
+                    occ=np.ascontiguousarray(self.occ), **self.reference_orbitals())
+
+    def reference_orbitals(self):
+        """Ground-state orbital of each propagated orbital (its starting
+        state, up to a global phase): real radial function phi [n_orb, N] in u
+        units with unit norm, its l and its
+        energy eps = sum_a lam_a |<q_a|phi>|^2 over the retained eigenpairs
+        (phi is an eigenvector of its block, so this is its eigenvalue)."""
+        orbs = self.atom.orbitals
+        phi = np.zeros((self.n_orb, self.N))
+        l0 = np.zeros(self.n_orb, dtype=np.int32)
+        eps = np.zeros(self.n_orb)
+        for k, (n, l, m, occ, s) in enumerate(orbs):
+            u = self.psi0[l, k]
+            # u = e^{i theta} v with v real: remove the global phase, unit norm
+            th = 0.5 * np.angle(np.sum(u * u))
+            v = (u * np.exp(-1j * th)).real
+            phi[k] = v / np.linalg.norm(v)
+            u = phi[k]
+            l0[k] = l
+            c = self.Q[l, :, :self.nkeep[l]].T @ u
+            eps[k] = float((self.lam[l, :self.nkeep[l]] * c * c).sum() / (c * c).sum())
+        return dict(phi=phi, l0=l0, eps=eps)
 
     def total_acceleration(self, acc, norm, E):
         """Ehrenfest dipole acceleration of all electrons,

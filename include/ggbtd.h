@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define GGB_ABI_VERSION 1
+#define GGB_ABI_VERSION 2
 
 #define GGB_OK 0
 #define GGB_E_ARG (-1)      /* invalid argument / size */
@@ -81,6 +81,13 @@ typedef struct ggb_static {
     const int32_t *m;          /* [n_orb] |m| of each orbital */
     const double *occ;         /* [n_orb] occupations */
     const double *rho_frozen;  /* [n_r] frozen-core u-unit density, or NULL */
+    /* FP32 mixed form (precision 1) only, else ignored; all three or NULL:
+     * the ground-state orbital each propagated orbital starts from, an
+     * eigenvector of its block's kinetic propagator.  The kinetic products
+     * act on psi minus its component along it, which is advanced in FP64. */
+    const double *phi;         /* [n_orb][n_r] radial function, u units, unit norm */
+    const int32_t *l0;         /* [n_orb] its l */
+    const double *eps;         /* [n_orb] its eigenvalue of H_l (Hartree) */
 } ggb_static;
 
 int ggb_abi_version(void);

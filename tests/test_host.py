@@ -99,3 +99,22 @@ def test_pulse_matches_oracle_and_configs():
     q = Pulse(1.5e14, 800.0, 8, second=(2, 1e-3, 0.7))
     Eq = OF.midpoint_series(q.E0, q.omega0, 8, 0.02, second=(2, 1e-3, 0.7))
     assert np.abs(q.midpoints(0.02) - Eq).max() < 1e-15
+
+
+@pytest.mark.parametrize("atom,N,R,lmax", [("He", 48, 30.0, 4), ("Ne", 48, 30.0, 4)])
+def test_reference_orbitals_are_unit_eigenvectors(atom, N, R, lmax):
+    # the FP32 mixed form advances each orbital's ground-state component
+    # exactly: phi must be the starting orbital up to a global phase, of unit
+    # norm, and an eigenvector of the retained spectral propagator
+    S = AtomSystem(atom, N, R, lmax, 0.05)
+    ref = S.reference_orbitals()
+    for k, (n, l, m, occ, s) in enumerate(S.atom.orbitals):
+        phi = ref["phi"][k]
+        assert ref["l0"][k] == l
+        assert abs(np.linalg.norm(phi) - 1.0) < 1e-14
+        assert abs(abs(np.vdot(phi, S.psi0[l, k])) - 1.0) < 1e-12
+        nk = S.nkeep[l]
+        c = S.Q[l, :, :nk].T @ phi
+        assert abs(c @ c - 1.0) < 1e-10          # inside the retained space
+        H = S.grid.hamiltonian(l, S.atom.Z, S.v_gs)
+        assert np.linalg.norm(H @ phi - ref["eps"][k] * phi) < 1e-6 * abs(ref["eps"][k])
