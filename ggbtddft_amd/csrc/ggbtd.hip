@@ -81,6 +81,29 @@ template <> struct mfma_t<float> {
     static __device__ __forceinline__ int row(int kr, int r) { return 4 * kr + r; }
 };
 
+// Work-groups of a launch are dispatched round-robin over the 8 XCDs, each
+// with its own L2.  xcd_linear renumbers them so that consecutive logical
+// work-groups run on the same XCD and share its L2 (a bijection for any grid).
+static __device__ __forceinline__ int xcd_linear() {
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int T = gx * gy * gridDim.z;
+    const int p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int q = T / 8, r = T % 8, x = p % 8;
+    return x * q + min(x, r) + p / 8;
+}
+struct tile3 {
+    int x, y, z;
+};
+// logical (x, y, z) of this work-group, x fastest
+static __device__ __forceinline__ tile3 xcd_tile() {
+    const int g = xcd_linear();
+    tile3 t;
+    t.x = g % gridDim.x;
+    t.y = (g / gridDim.x) % gridDim.y;
+    t.z = g / (gridDim.x * gridDim.y);
+    return t;
+}
+
 static __device__ __forceinline__ double c_lm(int l, int m) {
     // <l m|cos theta|l+1 m> (internal.py:80-84, angularCoefficients.cuh d_c)
     double dl = (double)l, dm = (double)m;
@@ -173,13 +196,14 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 int n_orb, double dt) {
     typedef typename mfma_t<T>::v4 v4;
     __shared__ double red[3][64][4];
-    const int l = blockIdx.z;
+    const tile3 tb = xcd_tile();  // (eigen tile, column tile, l): tiles of one l share psi
+    const int l = tb.z;
     const int nk = nkeep[l];
-    const int a0 = blockIdx.x * 16;
+    const int a0 = tb.x * 16;
     if (a0 >= nk) return;  // block-uniform
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kr = lane >> 4, lc = lane & 15;
-    const int col = blockIdx.y * 16 + lc;
+    const int col = tb.y * 16 + lc;
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
     const T *Ql = Q + (size_t)l * N * nk_max + a0 + lc;
@@ -248,13 +272,14 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
                                                 int C, int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
     __shared__ double red[3][64][4];
-    const int l = blockIdx.z;
+    const tile3 tb = xcd_tile();  // (radial tile, column tile, l): tiles of one l share Y
+    const int l = tb.z;
     const int nk = nkeep[l];
     const int kend = min(nk_max, (nk + 15) & ~15);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i0 = blockIdx.x * 16;
+    const int i0 = tb.x * 16;
     const int kr = lane >> 4, lc = lane & 15;
-    const int col = blockIdx.y * 16 + lc;
+    const int col = tb.y * 16 + lc;
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
     const int im_ = i0 + lc;
@@ -301,14 +326,36 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
 
 // ---------------------------------------------------------------------------
 // Tiled kinetic passes for many columns (instances sharing the tables).
-// Work-group tile 64 (M) x 64 (real columns), K chunks of 16 staged in LDS
-// with a register prefetch of the next chunk; wave w owns the 32 x 32
-// sub-tile (w >> 1, w & 1) as 2 x 2 MFMA tiles.  Same element-wise
-// summation order over K as the narrow kernels above (pass 1).
+// Work-group tile 64 (M) x 128 (real columns); wave w owns the 32 x 64
+// sub-tile (w >> 1, w & 1) as 2 x 4 MFMA tiles.  K advances in chunks of
+// kin_tk<T> (32 FP32 / 16 FP64 values) staged in LDS, double buffered, the
+// next chunk prefetched into registers while the current one is multiplied;
+// one barrier per chunk.  Rows of the LDS tiles are padded by 16 values so
+// that the four K rows a wave reads per product fall on distinct banks.
 // ---------------------------------------------------------------------------
-#define TB 64
-#define TK 16
-#define TLD (TB + 2)  // padded LDS row against bank conflicts
+#define TM 64
+#define TN 128
+template <typename T> struct kin_tk;
+template <> struct kin_tk<float> { static constexpr int v = 32; };
+template <> struct kin_tk<double> { static constexpr int v = 16; };
+#define TMP (TM + 16)
+#define TNP (TN + 16)
+
+// 8 products of one K step of 4 for the wave's 2 x 4 tiles
+template <typename T>
+static __device__ __forceinline__ void kin_tile_step(const T (*As)[TMP], const T (*Bs)[TNP], int kk,
+                                                     int wm, int wn, int lc,
+                                                     typename mfma_t<T>::v4 (&acc)[2][4]) {
+    const T a0 = As[kk][wm + lc], a1 = As[kk][wm + 16 + lc];
+    T b[4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) b[y] = Bs[kk][wn + 16 * y + lc];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        acc[0][y] = mfma_t<T>::op(a0, b[y], acc[0][y]);
+        acc[1][y] = mfma_t<T>::op(a1, b[y], acc[1][y]);
+    }
+}
 
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
@@ -318,77 +365,73 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
                                                       T *__restrict__ Y, int N, int nk_max,
                                                       int C, int n_orb, double dt) {
     typedef typename mfma_t<T>::v4 v4;
-    __shared__ T As[2][TK][TLD];  // [k][m]
-    __shared__ T Bs[2][TK][TLD];  // [k][n]
-    const int l = blockIdx.z;
+    constexpr int TKT = kin_tk<T>::v;
+    constexpr int AV = TKT * TM / WG;  // A values per thread (consecutive a)
+    constexpr int BV = TKT * 64 / WG;  // complex B values per thread (consecutive i)
+    __shared__ T As[2][TKT][TMP];      // [k = i][m = a]
+    __shared__ T Bs[2][TKT][TNP];      // [k = i][n = real column]
+    const tile3 tb = xcd_tile();       // (eigen tile, column tile, l)
+    const int l = tb.z;
     const int nk = nkeep[l];
-    const int a0 = blockIdx.x * TB;
+    const int a0 = tb.x * TM;
     if (a0 >= nk) return;  // block-uniform
-    const int n0 = blockIdx.y * TB;
+    const int n0 = tb.y * TN;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
     const T *Ql = Q + (size_t)l * N * nk_max;
-    const double *Xl = psi + (size_t)l * C * N * 2;
+    const double2 *Xl = (const double2 *)psi + (size_t)l * C * N;
     // global -> register mapping
-    const int ak = t >> 4, am = (t & 15) * 4;          // A: row k, 4 consecutive m
-    const int bc = t >> 3, bk = (t & 7) * 2;           // B: complex column, 2 consecutive k
+    const int ar = t / (TM / AV), am = (t % (TM / AV)) * AV;  // A: row i, AV consecutive a
+    const int bc = t / (TKT / BV), bk = (t % (TKT / BV)) * BV;  // B: complex column, BV consecutive i
     const int cg = (n0 >> 1) + bc;
     const bool cok = cg < C;
+    const bool aok = a0 + am < nk_max;  // nk_max is a multiple of 16 >= AV: all or none
     double rcr = 0.0, rci = 0.0;
     const double *rrow = (mfma_t<T>::mixed && cok) ? kin_ref_row(ref, l, cg, 0, N, n_orb, false, &rcr) : nullptr;
     if (rrow) kin_ref_row(ref, l, cg, 1, N, n_orb, false, &rci);
-    T ra[4], rb[4];
+    T ra[AV], rb[2 * BV];
     auto load = [&](int k0) {
-        const int i = k0 + ak;
+        const int i = k0 + ar;
+        const bool ok = aok && i < N;
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            ra[u] = (i < N && a0 + am + u < nk_max) ? Ql[(size_t)i * nk_max + a0 + am + u] : (T)0;
+        for (int u = 0; u < AV; ++u) ra[u] = ok ? Ql[(size_t)i * nk_max + a0 + am + u] : (T)0;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < BV; ++u) {
             const int ii = k0 + bk + u;
-            const bool ok = cok && ii < N;
-            double xr = ok ? Xl[((size_t)cg * N + ii) * 2] : 0.0;
-            double xi = ok ? Xl[((size_t)cg * N + ii) * 2 + 1] : 0.0;
-            if (rrow && ok) {
-                xr -= rcr * rrow[ii];
-                xi -= rci * rrow[ii];
+            const bool okb = cok && ii < N;
+            double2 v = okb ? Xl[(size_t)cg * N + ii] : make_double2(0.0, 0.0);
+            if (rrow && okb) {
+                v.x -= rcr * rrow[ii];
+                v.y -= rci * rrow[ii];
             }
-            rb[2 * u] = (T)xr;
-            rb[2 * u + 1] = (T)xi;
+            rb[2 * u] = (T)v.x;
+            rb[2 * u + 1] = (T)v.y;
         }
     };
     auto stash = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) As[buf][ak][am + u] = ra[u];
+        for (int u = 0; u < AV; ++u) As[buf][ar][am + u] = ra[u];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < BV; ++u) {
             Bs[buf][bk + u][2 * bc] = rb[2 * u];
             Bs[buf][bk + u][2 * bc + 1] = rb[2 * u + 1];
         }
     };
-    v4 acc[2][2];
+    v4 acc[2][4];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = (v4){0, 0, 0, 0};
+        for (int y = 0; y < 4; ++y) acc[x][y] = (v4){0, 0, 0, 0};
     load(0);
     stash(0);
     __syncthreads();
     int buf = 0;
-    for (int k0 = 0; k0 < N; k0 += TK) {
-        const bool more = k0 + TK < N;
-        if (more) load(k0 + TK);
+    for (int k0 = 0; k0 < N; k0 += TKT) {
+        const bool more = k0 + TKT < N;
+        if (more) load(k0 + TKT);
 #pragma unroll
-        for (int ks = 0; ks < TK / 4; ++ks) {
-            const int kk = ks * 4 + kr;
-            const T a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
-            const T b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
-            acc[0][0] = mfma_t<T>::op(a0v, b0v, acc[0][0]);
-            acc[0][1] = mfma_t<T>::op(a0v, b1v, acc[0][1]);
-            acc[1][0] = mfma_t<T>::op(a1v, b0v, acc[1][0]);
-            acc[1][1] = mfma_t<T>::op(a1v, b1v, acc[1][1]);
-        }
+        for (int ks = 0; ks < TKT / 4; ++ks) kin_tile_step<T>(As[buf], Bs[buf], ks * 4 + kr, wm, wn, lc, acc);
         if (more) stash(buf ^ 1);
         __syncthreads();
         buf ^= 1;
@@ -397,7 +440,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
+        for (int y = 0; y < 4; ++y) {
             const int col = n0 + wn + 16 * y + lc;
             const int c = col >> 1, p = col & 1;
             const int ab = a0 + wm + 16 * x;
@@ -428,64 +471,57 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
                                                       double *__restrict__ psi, int N, int nk_max,
                                                       int C, int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
-    __shared__ T As[2][TK][TLD];  // [k = a][m = i]
-    __shared__ T Bs[2][TK][TLD];  // [k = a][n = col]
-    const int l = blockIdx.z;
+    constexpr int TKT = kin_tk<T>::v;
+    constexpr int AV = TKT * TM / WG;  // A values per thread (consecutive a of one row i)
+    constexpr int BV = TKT * TN / WG;  // B values per thread (consecutive a of one column)
+    __shared__ T As[2][TKT][TMP];      // [k = a][m = i]
+    __shared__ T Bs[2][TKT][TNP];      // [k = a][n = real column]
+    const tile3 tb = xcd_tile();       // (radial tile, column tile, l)
+    const int l = tb.z;
     const int nk = nkeep[l];
-    const int i0 = blockIdx.x * TB;
-    const int n0 = blockIdx.y * TB;
+    const int i0 = tb.x * TM;
+    const int n0 = tb.y * TN;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
     const T *Ql = Q + (size_t)l * N * nk_max;
     const T *Yl = Y + (size_t)l * C * 2 * nk_max;
-    // A: Q[i0 + am][k0 + ak .. +4]  (row i, 4 consecutive a); B: Y[col][k0 + bk .. +4]
-    const int am = t >> 2, ak = (t & 3) * 4;
-    const int bn = t >> 2, bk = (t & 3) * 4;
+    const int am = t / (TKT / AV), ak = (t % (TKT / AV)) * AV;  // A: row i, AV consecutive a
+    const int bn = t / (TKT / BV), bk = (t % (TKT / BV)) * BV;  // B: real column, BV consecutive a
     const int ia = i0 + am;
     const int colb = n0 + bn;
     const bool bok = (colb >> 1) < C;
-    T ra[4], rb[4];
+    // the tables and Y are zero from nkeep up to nk_max: runs of AV, BV (<= 16,
+    // dividing 16) are wholly inside or outside [0, nk_max)
+    T ra[AV], rb[BV];
     auto load = [&](int k0) {
+        const bool oka = ia < N && k0 + ak < nk_max;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int a = k0 + ak + u;
-            ra[u] = (ia < N && a < nk) ? Ql[(size_t)ia * nk_max + a] : (T)0;
-            const int ab = k0 + bk + u;
-            rb[u] = (bok && ab < nk) ? Yl[(size_t)colb * nk_max + ab] : (T)0;
-        }
+        for (int u = 0; u < AV; ++u) ra[u] = oka ? Ql[(size_t)ia * nk_max + k0 + ak + u] : (T)0;
+        const bool okb = bok && k0 + bk < nk_max;
+#pragma unroll
+        for (int u = 0; u < BV; ++u) rb[u] = okb ? Yl[(size_t)colb * nk_max + k0 + bk + u] : (T)0;
     };
     auto stash = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            As[buf][ak + u][am] = ra[u];
-            Bs[buf][bk + u][bn] = rb[u];
-        }
+        for (int u = 0; u < AV; ++u) As[buf][ak + u][am] = ra[u];
+#pragma unroll
+        for (int u = 0; u < BV; ++u) Bs[buf][bk + u][bn] = rb[u];
     };
-    v4 acc[2][2];
+    v4 acc[2][4];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = (v4){0, 0, 0, 0};
+        for (int y = 0; y < 4; ++y) acc[x][y] = (v4){0, 0, 0, 0};
     load(0);
     stash(0);
     __syncthreads();
     int buf = 0;
-    for (int k0 = 0; k0 < nk; k0 += TK) {
-        const bool more = k0 + TK < nk;
-        if (more) load(k0 + TK);
+    for (int k0 = 0; k0 < nk; k0 += TKT) {
+        const bool more = k0 + TKT < nk;
+        if (more) load(k0 + TKT);
 #pragma unroll
-        for (int ks = 0; ks < TK / 4; ++ks) {
-            // lane kr of MFMA ks takes a = k0 + 4 kr + ks: the K order of k_kin_bwd,
-            // so an instance's result does not depend on which kernel serves it
-            const int kk = 4 * kr + ks;
-            const T a0v = As[buf][kk][wm + lc], a1v = As[buf][kk][wm + 16 + lc];
-            const T b0v = Bs[buf][kk][wn + lc], b1v = Bs[buf][kk][wn + 16 + lc];
-            acc[0][0] = mfma_t<T>::op(a0v, b0v, acc[0][0]);
-            acc[0][1] = mfma_t<T>::op(a0v, b1v, acc[0][1]);
-            acc[1][0] = mfma_t<T>::op(a1v, b0v, acc[1][0]);
-            acc[1][1] = mfma_t<T>::op(a1v, b1v, acc[1][1]);
-        }
+        for (int ks = 0; ks < TKT / 4; ++ks) kin_tile_step<T>(As[buf], Bs[buf], ks * 4 + kr, wm, wn, lc, acc);
         if (more) stash(buf ^ 1);
         __syncthreads();
         buf ^= 1;
@@ -494,7 +530,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
+        for (int y = 0; y < 4; ++y) {
             const int col = n0 + wn + 16 * y + lc;
             const int c = col >> 1, p = col & 1;
             if (c >= C) continue;
@@ -672,9 +708,13 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     __shared__ double a_i[RT];
     __shared__ double red[3][ANG_MAXW];
 
-    const int tile_i = blockIdx.x, c = blockIdx.y;
+    // logical order (orbital, radial tile, instance), orbital fastest: the
+    // orbitals of one instance and radial tile share the phase factors in L2
+    const int lg = xcd_linear();
+    const int korb = lg % g.n_orb, rest = lg / g.n_orb;
+    const int tile_i = rest % gridDim.x, inst = rest / gridDim.x;
+    const int c = inst * g.n_orb + korb;
     const int i0 = tile_i * RT;
-    const int inst = c / g.n_orb;
     const int m = g.mcol[c % g.n_orb];
     const int t = threadIdx.x, nthr = blockDim.x;
     const int wave = t >> 6, lane = t & 63;
@@ -702,9 +742,9 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     // load the l-space tile
     for (int idx = t; idx < g.Nlx * RT; idx += nthr) {
         const int l = idx / RT, il = idx % RT;
-        const int i = i0 + il;
+        const int ii = i0 + il;
         double2 v = make_double2(0.0, 0.0);
-        if (l < g.Nl && i < g.N) v = ((const double2 *)g.psi)[((size_t)l * g.C + c) * g.N + i];
+        if (l < g.Nl && ii < g.N) v = ((const double2 *)g.psi)[((size_t)l * g.C + c) * g.N + ii];
         X[l * 16 + 2 * il] = v.x;
         X[l * 16 + 2 * il + 1] = v.y;
     }
@@ -1582,12 +1622,12 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
         k_kin_ref<<<ctx->C, WG, 0, ctx->stream>>>(ctx->psi, ctx->phi_ref, ctx->l0, ctx->eps, ctx->ab,
                                                   p.n_r, ctx->C, p.n_orb, p.dt);
     if (ctx->C > KIN_NARROW_MAX_C) {
-        const int nct = (2 * ctx->C + TB - 1) / TB;
+        const int nct = (2 * ctx->C + TN - 1) / TN;
         if (pass == 0)
-            k_kin_fwd_tiled<T><<<dim3((p.nk_max + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
+            k_kin_fwd_tiled<T><<<dim3((p.nk_max + TM - 1) / TM, nct, p.n_l), WG, 0, ctx->stream>>>(
                 Q, ctx->lam, ctx->nkeep, ctx->psi, ref, Y, p.n_r, p.nk_max, ctx->C, p.n_orb, p.dt);
         else
-            k_kin_bwd_tiled<T><<<dim3((p.n_r + TB - 1) / TB, nct, p.n_l), WG, 0, ctx->stream>>>(
+            k_kin_bwd_tiled<T><<<dim3((p.n_r + TM - 1) / TM, nct, p.n_l), WG, 0, ctx->stream>>>(
                 Q, ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max, ctx->C, p.n_orb);
         return;
     }
