@@ -165,6 +165,26 @@ __global__ __launch_bounds__(WG) void k_kin_ref(const double *__restrict__ psi,
     }
 }
 
+// FP32 copy of the state for the tiled pass 1, the ground-state component
+// alpha phi taken out (mixed form): psi32[l][c][i] = psi - [l = l0] alpha phi.
+__global__ __launch_bounds__(WG) void k_kin_prep(const double2 *__restrict__ psi, KinRef ref,
+                                                 float2 *__restrict__ psi32, int N, int C,
+                                                 int n_orb) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    const int c = blockIdx.y, l = blockIdx.z;
+    if (i >= N) return;
+    const size_t o = ((size_t)l * C + c) * N + i;
+    double2 v = psi[o];
+    const int k = c % n_orb;
+    if (ref.phi && ref.l0[k] == l) {
+        const double4 a = ref.ab[c];
+        const double f = ref.phi[(size_t)k * N + i];
+        v.x -= a.x * f;
+        v.y -= a.y * f;
+    }
+    psi32[o] = make_float2((float)v.x, (float)v.y);
+}
+
 // reference component of lane column (c, p) in block l: coefficient (alpha or
 // beta, component p) and phi row, or a null row
 static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, int l, int c, int p,
@@ -361,9 +381,11 @@ template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
                                                       const double *__restrict__ lam,
                                                       const int32_t *__restrict__ nkeep,
-                                                      const double *__restrict__ psi, KinRef ref,
+                                                      const double *__restrict__ psi,
+                                                      const float2 *__restrict__ psi32, KinRef ref,
                                                       T *__restrict__ Y, int N, int nk_max,
                                                       int C, int n_orb, double dt) {
+    // FP32 (mixed): the state comes from psi32 (k_kin_prep), FP64: from psi
     typedef typename mfma_t<T>::v4 v4;
     constexpr int TKT = kin_tk<T>::v;
     constexpr int AV = TKT * TM / WG;  // A values per thread (consecutive a)
@@ -387,9 +409,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
     const int cg = (n0 >> 1) + bc;
     const bool cok = cg < C;
     const bool aok = a0 + am < nk_max;  // nk_max is a multiple of 16 >= AV: all or none
-    double rcr = 0.0, rci = 0.0;
-    const double *rrow = (mfma_t<T>::mixed && cok) ? kin_ref_row(ref, l, cg, 0, N, n_orb, false, &rcr) : nullptr;
-    if (rrow) kin_ref_row(ref, l, cg, 1, N, n_orb, false, &rci);
+    const float2 *Xs = psi32 + ((size_t)l * C + (cok ? cg : 0)) * N;
     T ra[AV], rb[2 * BV];
     auto load = [&](int k0) {
         const int i = k0 + ar;
@@ -400,13 +420,15 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
         for (int u = 0; u < BV; ++u) {
             const int ii = k0 + bk + u;
             const bool okb = cok && ii < N;
-            double2 v = okb ? Xl[(size_t)cg * N + ii] : make_double2(0.0, 0.0);
-            if (rrow && okb) {
-                v.x -= rcr * rrow[ii];
-                v.y -= rci * rrow[ii];
+            if (mfma_t<T>::mixed) {
+                const float2 v = okb ? Xs[ii] : make_float2(0.f, 0.f);
+                rb[2 * u] = (T)v.x;
+                rb[2 * u + 1] = (T)v.y;
+            } else {
+                const double2 v = okb ? Xl[(size_t)cg * N + ii] : make_double2(0.0, 0.0);
+                rb[2 * u] = (T)v.x;
+                rb[2 * u + 1] = (T)v.y;
             }
-            rb[2 * u] = (T)v.x;
-            rb[2 * u + 1] = (T)v.y;
         }
     };
     auto stash = [&](int buf) {
@@ -1191,6 +1213,7 @@ struct ggb_ctx {
     double2 *phase;
     // mixed form: ground-state reference of each orbital and its overlaps
     bool have_ref;
+    float2 *psi32;  // FP32 copy of the state for the tiled pass 1 (mixed form, C > 8)
     double *phi_ref, *eps;
     int32_t *l0;
     double4 *ab;
@@ -1252,7 +1275,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
                     ctx->r, ctx->lr, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
-                    ctx->l0, ctx->ab};
+                    ctx->l0, ctx->ab, ctx->psi32};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
@@ -1278,7 +1301,11 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     d += 2 * nI * N * Nth;               // phase (double2)
     d += 2 * nI * nh * N;                // q, V
     d += C * (size_t)ntiles * 3;         // obs_part
-    return d * sizeof(double) + tbytes + (Nl + p->n_orb) * sizeof(int32_t);
+    if (p->precision) {
+        d += (size_t)p->n_orb * (N + 1) + 4 * C;                 // phi_ref, eps, ab
+        if (C > KIN_NARROW_MAX_C) tbytes += Nl * C * N * 2 * 4;  // psi32
+    }
+    return d * sizeof(double) + tbytes + (Nl + 2 * p->n_orb) * sizeof(int32_t);
 }
 
 extern "C" int ggb_footprint(const ggb_problem *p, uint64_t *bytes) {
@@ -1323,6 +1350,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->dens = ctx->vxc = ctx->rho_u = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
     ctx->phase = nullptr;
     ctx->have_ref = false;
+    ctx->psi32 = nullptr;
     ctx->phi_ref = ctx->eps = nullptr;
     ctx->l0 = nullptr;
     ctx->ab = nullptr;
@@ -1380,6 +1408,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
         ok &= dalloc(&ctx->eps, (size_t)p->n_orb) == hipSuccess;
         ok &= dalloc(&ctx->l0, (size_t)p->n_orb) == hipSuccess;
         ok &= dalloc(&ctx->ab, C) == hipSuccess;
+        if (C > KIN_NARROW_MAX_C) ok &= dalloc(&ctx->psi32, Nl * C * N) == hipSuccess;
     }
     if (!ok) return bail("ggb_create: device allocation failed", GGB_E_NOMEM);
     // zero everything that kernels may read before it is written
@@ -1623,9 +1652,13 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
                                                   p.n_r, ctx->C, p.n_orb, p.dt);
     if (ctx->C > KIN_NARROW_MAX_C) {
         const int nct = (2 * ctx->C + TN - 1) / TN;
+        if (pass == 0 && mfma_t<T>::mixed)
+            k_kin_prep<<<dim3((p.n_r + WG - 1) / WG, ctx->C, p.n_l), WG, 0, ctx->stream>>>(
+                (const double2 *)ctx->psi, ref, ctx->psi32, p.n_r, ctx->C, p.n_orb);
         if (pass == 0)
             k_kin_fwd_tiled<T><<<dim3((p.nk_max + TM - 1) / TM, nct, p.n_l), WG, 0, ctx->stream>>>(
-                Q, ctx->lam, ctx->nkeep, ctx->psi, ref, Y, p.n_r, p.nk_max, ctx->C, p.n_orb, p.dt);
+                Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->psi32, ref, Y, p.n_r, p.nk_max, ctx->C,
+                p.n_orb, p.dt);
         else
             k_kin_bwd_tiled<T><<<dim3((p.n_r + TM - 1) / TM, nct, p.n_l), WG, 0, ctx->stream>>>(
                 Q, ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max, ctx->C, p.n_orb);
