@@ -85,6 +85,19 @@ def batched_run(sysm, pulse, args, fp64_value):
             "ratio_vs_fp64_plain": val / fp64_value}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
+    calibration); None when the summary is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "pmc_traffic_argon.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def precision_report(sysm, pulse):
     """Full-length runs in FP64 and FP32; errors of the FP32 spectrum."""
     from ggbtddft_amd import spectrum as sp
@@ -204,6 +217,7 @@ def main():
     avg_qt = ms_qt / max(n_qt, 1)
     avg_q = ms_q / max(n_q, 1)
     achieved = bytes_qt / (avg_qt * 1e-3) / 1e9
+    traffic = measured_traffic()
     line = {
         "metric": "orbital-timesteps/s, Ar N=512 Lmax=64: batched FP32 vs FP64; p99 time-to-spectrum",
         "value": value,
@@ -225,7 +239,9 @@ def main():
         "phase_ms_per_step": phases,
         "roofline": {"kernel": "k_kin_fwd (kinetic propagator, Q^T pass)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "bytes_per_launch": bytes_qt, "avg_launch_ms": avg_qt,
                      "avg_launch_ms_q_pass": avg_q},
     }
