@@ -147,7 +147,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--batched", type=int, default=32,
+    ap.add_argument("--batched", type=int, default=64,
                     help="Ar instances for the batched FP32 leg (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--precision-report", action="store_true",

@@ -59,9 +59,10 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise GGBError("libggbtd.so not built (%s); run ggbtddft_amd.build.build()" % LIB_PATH)
-    lib = ctypes.CDLL(LIB_PATH)
+    path = os.environ.get("GGBTD_LIB", LIB_PATH)  # a differently built copy, for comparisons
+    if not os.path.exists(path):
+        raise GGBError("libggbtd.so not built (%s); run ggbtddft_amd.build.build()" % path)
+    lib = ctypes.CDLL(path)
     vp = ctypes.c_void_p
     lib.ggb_abi_version.restype = ctypes.c_int
     lib.ggb_abi_version.argtypes = []

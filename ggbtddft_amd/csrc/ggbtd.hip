@@ -49,6 +49,7 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 #define RT 8          // radial nodes per angular tile (16 real columns)
 #define RHO_MIN 1e-40 // v_xc = 0 at or below this density
 #define KIN_NARROW_MAX_C 8  // up to 8 complex columns: one 16-wide MFMA column tile
+#define TIMING_STRIDE 16    // timing mode 1 samples every 16th step
 
 static __device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -104,20 +105,13 @@ static __device__ __forceinline__ tile3 xcd_tile() {
     return t;
 }
 
-static __device__ __forceinline__ double c_lm(int l, int m) {
-    // <l m|cos theta|l+1 m> (internal.py:80-84, angularCoefficients.cuh d_c)
-    double dl = (double)l, dm = (double)m;
-    return sqrt(((dl + 1.0) * (dl + 1.0) - dm * dm) / ((2.0 * dl + 1.0) * (2.0 * dl + 3.0)));
-}
 
 // component p of e^{-i lam dt} (v_re + i v_im), from this lane's v and its
-// partner's vo
+// partner's vo; f = (cos(lam dt), sin(lam dt)) from the table built at upload
 template <typename T>
-static __device__ __forceinline__ T kin_factor(double v, double vo, int p, double lam, double dt) {
+static __device__ __forceinline__ T kin_factor(double v, double vo, int p, double2 f) {
     const double re = p ? vo : v, im = p ? v : vo;
-    double sn, cs;
-    sincos(lam * dt, &sn, &cs);
-    return (T)(p ? (cs * im - sn * re) : (cs * re + sn * im));
+    return (T)(p ? (f.x * im - f.y * re) : (f.x * re + f.y * im));
 }
 
 // Ground-state reference of the mixed form: for column c (orbital k = c mod
@@ -209,11 +203,11 @@ static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, in
 #define KU 8
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
-                                                const double *__restrict__ lam,
+                                                const double2 *__restrict__ kph,
                                                 const int32_t *__restrict__ nkeep,
                                                 const double *__restrict__ psi, KinRef ref,
                                                 T *__restrict__ Y, int N, int nk_max, int C,
-                                                int n_orb, double dt) {
+                                                int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
     __shared__ double red[3][64][4];
     const tile3 tb = xcd_tile();  // (eigen tile, column tile, l): tiles of one l share psi
@@ -256,14 +250,14 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 #pragma unroll
     for (int r = 0; r < 4; ++r)
         v4d[r] = (((double)acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
-    const double *laml = lam + (size_t)l * nk_max;
+    const double2 *kphl = kph + (size_t)l * nk_max;
     T o[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int a = a0 + mfma_t<T>::row(kr, r);
         const double v = v4d[r];
         const double vo = __shfl_xor(v, 1);
-        o[r] = kin_factor<T>(v, vo, p, laml[a < nk ? a : 0], dt);
+        o[r] = kin_factor<T>(v, vo, p, kphl[a < nk ? a : 0]);
     }
     if (colok) {
         T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max + a0;
@@ -348,7 +342,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
 // Tiled kinetic passes for many columns (instances sharing the tables).
 // Work-group tile 64 (M) x 128 (real columns); wave w owns the 32 x 64
 // sub-tile (w >> 1, w & 1) as 2 x 4 MFMA tiles.  K advances in chunks of
-// kin_tk<T> (32 FP32 / 16 FP64 values) staged in LDS, double buffered, the
+// kin_tk<T> (16 values) staged in LDS, double buffered, the
 // next chunk prefetched into registers while the current one is multiplied;
 // one barrier per chunk.  Rows of the LDS tiles are padded by 16 values so
 // that the four K rows a wave reads per product fall on distinct banks.
@@ -356,7 +350,10 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
 #define TM 64
 #define TN 128
 template <typename T> struct kin_tk;
-template <> struct kin_tk<float> { static constexpr int v = 32; };
+#ifndef KIN_TK_F32
+#define KIN_TK_F32 16  // measured: 16 beats 32 (more work-groups per CU, 28 KB of LDS each)
+#endif
+template <> struct kin_tk<float> { static constexpr int v = KIN_TK_F32; };
 template <> struct kin_tk<double> { static constexpr int v = 16; };
 #define TMP (TM + 16)
 #define TNP (TN + 16)
@@ -379,12 +376,12 @@ static __device__ __forceinline__ void kin_tile_step(const T (*As)[TMP], const T
 
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
-                                                      const double *__restrict__ lam,
+                                                      const double2 *__restrict__ kph,
                                                       const int32_t *__restrict__ nkeep,
                                                       const double *__restrict__ psi,
                                                       const float2 *__restrict__ psi32, KinRef ref,
                                                       T *__restrict__ Y, int N, int nk_max,
-                                                      int C, int n_orb, double dt) {
+                                                      int C, int n_orb) {
     // FP32 (mixed): the state comes from psi32 (k_kin_prep), FP64: from psi
     typedef typename mfma_t<T>::v4 v4;
     constexpr int TKT = kin_tk<T>::v;
@@ -458,7 +455,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
         __syncthreads();
         buf ^= 1;
     }
-    const double *laml = lam + (size_t)l * nk_max;
+    const double2 *kphl = kph + (size_t)l * nk_max;
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
 #pragma unroll
@@ -472,7 +469,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
                 const int a = ab + mfma_t<T>::row(kr, r);
                 const double v = (double)acc[x][y][r];
                 const double vo = __shfl_xor(v, 1);
-                o[r] = kin_factor<T>(v, vo, p, laml[a < nk ? a : 0], dt);
+                o[r] = kin_factor<T>(v, vo, p, kphl[a < nk ? a : 0]);
             }
             if (c < C) {
                 T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max;
@@ -614,6 +611,13 @@ struct AngArgs {
     double dt;
 };
 
+static __device__ __forceinline__ double c_lm(int l, int m) {
+    // <l m|cos theta|l+1 m> (internal.py:80-84, angularCoefficients.cuh d_c); computed
+    // in registers rather than read from a table: the laser sits on the latency path
+    const double dl = (double)l, dm = (double)m;
+    return sqrt(((dl + 1.0) * (dl + 1.0) - dm * dm) / ((2.0 * dl + 1.0) * (2.0 * dl + 3.0)));
+}
+
 static __device__ void ang_laser(double *X, int parity, int m, int Nl, const double *a_i) {
     // pairs (l, l+1), l = m + parity + 2 q, l + 1 < Nl; thread -> (q, i_local)
     const int t = threadIdx.x;
@@ -654,26 +658,18 @@ static __device__ void ang_fwd(const T *Bs, const T *Ba, const double *X, const 
                                typename mfma_t<T>::v4 &o) {
     typedef typename mfma_t<T>::v4 v4;
     const int lane = threadIdx.x & 63, kr = lane >> 4, lc = lane & 15;
-    const int j = tile * 16 + lc;
-    const bool jok = j < g.hq;
-    const T *As = Bs + (size_t)(jok ? j : 0) * g.sS;
-    const T *Aa = Ba + (size_t)(jok ? j : 0) * g.sA;
+    // rows j >= hq read row hq - 1: their results are never used (j >= h);
+    // table columns k >= ns are zero and X rows up to Nl + 5 exist (zero), so
+    // the loops need no per-element guards
+    const int j = min(tile * 16 + lc, g.hq - 1);
+    const T *As = Bs + (size_t)j * g.sS;
+    const T *Aa = Ba + (size_t)j * g.sA;
     v4 ae = (v4){0, 0, 0, 0}, ao = (v4){0, 0, 0, 0};
     const int kmax = pr.ks > pr.ka ? pr.ks : pr.ka;
     for (int k0 = 0; k0 < kmax; k0 += 4) {
         const int k = k0 + kr;
-        if (k0 < pr.ks) {
-            const int l = m + 2 * k;
-            const T a = jok ? As[k] : (T)0;
-            const T b = l < g.Nl ? (T)X[l * 16 + lc] : (T)0;
-            ae = mfma_t<T>::op(a, b, ae);
-        }
-        if (k0 < pr.ka) {
-            const int l = m + 1 + 2 * k;
-            const T a = jok ? Aa[k] : (T)0;
-            const T b = l < g.Nl ? (T)X[l * 16 + lc] : (T)0;
-            ao = mfma_t<T>::op(a, b, ao);
-        }
+        if (k0 < pr.ks) ae = mfma_t<T>::op(As[k], (T)X[(m + 2 * k) * 16 + lc], ae);
+        if (k0 < pr.ka) ao = mfma_t<T>::op(Aa[k], (T)X[(m + 1 + 2 * k) * 16 + lc], ao);
     }
     e = ae;
     o = ao;
@@ -694,22 +690,18 @@ static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D,
     const int st = sym ? g.sS : g.sA;
     const int kk = sym ? pr.ks : pr.ka;
     const int nn = sym ? pr.ns : pr.na;
-    const int k = tile * 16 + lc;
-    const bool kok = k < kk;
-    const T *Ac = A + (kok ? k : 0);
+    // output rows k >= kk read column kk - 1 (in range); they are not stored
+    const T *Ac = A + min(tile * 16 + lc, kk - 1);
     v4 a0 = (v4){0, 0, 0, 0}, a1 = (v4){0, 0, 0, 0};
     int j0 = 0;
     for (; j0 + 8 <= g.hq; j0 += 8) {
         const int ja = j0 + kr, jb = j0 + 4 + kr;
-        const T va = kok ? Ac[(size_t)ja * st] : (T)0;
-        const T vb = kok ? Ac[(size_t)jb * st] : (T)0;
-        a0 = mfma_t<T>::op(va, Pv[ja * 16 + lc], a0);
-        a1 = mfma_t<T>::op(vb, Pv[jb * 16 + lc], a1);
+        a0 = mfma_t<T>::op(Ac[(size_t)ja * st], Pv[ja * 16 + lc], a0);
+        a1 = mfma_t<T>::op(Ac[(size_t)jb * st], Pv[jb * 16 + lc], a1);
     }
     if (j0 < g.hq) {
         const int ja = j0 + kr;
-        const T va = kok ? Ac[(size_t)ja * st] : (T)0;
-        a0 = mfma_t<T>::op(va, Pv[ja * 16 + lc], a0);
+        a0 = mfma_t<T>::op(Ac[(size_t)ja * st], Pv[ja * 16 + lc], a0);
     }
     const int off = sym ? 0 : 1;
 #pragma unroll
@@ -1190,7 +1182,7 @@ static ang_geom ang_geometry(const ggb_problem *p) {
     a.nblk = ((a.hq * (a.sS + a.sA) + per16 - 1) / per16) * per16;
     const int mt = (a.h + 15) / 16, nb = (ns0 + 15) / 16 + (na0 + 15) / 16;
     a.waves = mt > nb ? mt : nb;
-    const size_t tiles = (size_t)p->n_l * 16 * sizeof(double) + (size_t)2 * a.hq * 16 * es;
+    const size_t tiles = (size_t)(p->n_l + 6) * 16 * sizeof(double) + (size_t)2 * a.hq * 16 * es;
     const size_t stat = (RT + 3 * ANG_MAXW) * sizeof(double);
     a.glb = tiles + (size_t)a.nblk * es + stat > ANG_LDS_MAX;
     a.lds = a.glb ? tiles : tiles + (size_t)a.nblk * es;
@@ -1207,8 +1199,9 @@ struct ggb_ctx {
     int n_field;
     bool field_ring;  // ggb_field_ring: field index = step mod n_field
     // device buffers
-    double *psi, *Y, *Q, *lam, *B, *PL, *PLT, *chi, *omega, *r, *lr, *wr, *v_gs, *mask, *occ, *rho_frozen;
+    double *psi, *Y, *Q, *B, *PL, *PLT, *chi, *omega, *r, *lr, *wr, *v_gs, *mask, *occ, *rho_frozen;
     int32_t *nkeep, *mcol;
+    double2 *kph;  // [n_l][nk_max] (cos, sin)(lam dt) of the kinetic propagator
     double *dens, *vxc, *rho_u, *q, *V, *dv, *obs_part, *obs, *E;
     double2 *phase;
     // mixed form: ground-state reference of each orbital and its overlaps
@@ -1271,7 +1264,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->p.device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->lam, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
+    void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->kph, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
                     ctx->r, ctx->lr, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
@@ -1293,7 +1286,7 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     const size_t es = p->precision ? 4 : 8;
     const size_t nblk = ang_geometry(p).nblk;
     size_t tbytes = (Nl * C * nk * 2 + Nl * N * nk + (size_t)p->n_m * nblk) * es;  // Y, Q, B
-    d += Nl * nk;                        // lam
+    d += 2 * Nl * nk;                    // e^{-i lam dt}
     d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
     d += 6 * N + p->n_orb;               // r, lr, wr, v_gs, mask, rho_frozen, occ
     d += C * N * Nth;                    // dens
@@ -1344,7 +1337,8 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->n_field = 0;
     ctx->field_ring = false;
     ctx->obs_cap = 0;
-    ctx->psi = ctx->Y = ctx->Q = ctx->lam = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
+    ctx->kph = nullptr;
+    ctx->psi = ctx->Y = ctx->Q = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
     ctx->r = ctx->lr = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
     ctx->nkeep = ctx->mcol = nullptr;
     ctx->dens = ctx->vxc = ctx->rho_u = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
@@ -1381,7 +1375,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     const size_t nB = (size_t)p->n_m * ctx->ang.nblk;
     ok &= dalloc(&ctx->Y, (nY * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->Q, (nQ * es + 7) / 8) == hipSuccess;
-    ok &= dalloc(&ctx->lam, Nl * (size_t)p->nk_max) == hipSuccess;
+    ok &= dalloc(&ctx->kph, Nl * (size_t)p->nk_max) == hipSuccess;
     ok &= dalloc(&ctx->nkeep, Nl) == hipSuccess;
     ok &= dalloc(&ctx->B, (nB * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->PL, Nth * (size_t)p->n_har) == hipSuccess;
@@ -1413,7 +1407,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     if (!ok) return bail("ggb_create: device allocation failed", GGB_E_NOMEM);
     // zero everything that kernels may read before it is written
     if (hipMemset(ctx->B, 0, nB * es) != hipSuccess || hipMemset(ctx->Q, 0, nQ * es) != hipSuccess ||
-        hipMemset(ctx->lam, 0, Nl * (size_t)p->nk_max * sizeof(double)) != hipSuccess ||
+        hipMemset(ctx->kph, 0, Nl * (size_t)p->nk_max * sizeof(double2)) != hipSuccess ||
         hipMemset(ctx->Y, 0, nY * es) != hipSuccess ||
         hipMemset(ctx->psi, 0, Nl * C * N * 2 * sizeof(double)) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess)
@@ -1458,7 +1452,16 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
         for (size_t k = 0; k < q32.size(); ++k) q32[k] = (float)s->Q[k];
         HIPCHK(ctx, hipMemcpy(ctx->Q, q32.data(), q32.size() * 4, hipMemcpyHostToDevice));
     }
-    HIPCHK(ctx, hipMemcpyAsync(ctx->lam, s->lam, Nl * p.nk_max * 8, hipMemcpyHostToDevice, st));
+    {
+        // e^{-i lam dt} of every retained eigenvalue, in FP64 on the host
+        std::vector<double2> f(Nl * p.nk_max, make_double2(1.0, 0.0));
+        for (size_t l = 0; l < Nl; ++l)
+            for (int a = 0; a < s->nkeep[l]; ++a) {
+                const double x = s->lam[l * p.nk_max + a] * p.dt;
+                f[l * p.nk_max + a] = make_double2(std::cos(x), std::sin(x));
+            }
+        HIPCHK(ctx, hipMemcpy(ctx->kph, f.data(), f.size() * sizeof(double2), hipMemcpyHostToDevice));
+    }
     HIPCHK(ctx, hipMemcpyAsync(ctx->nkeep, s->nkeep, Nl * 4, hipMemcpyHostToDevice, st));
     {
         // parity half tables of each m (see the angular kernel): rows j < h of
@@ -1549,7 +1552,7 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
     g.obs_part = ctx->obs_part;
     g.N = ctx->p.n_r;
     g.Nl = ctx->p.n_l;
-    g.Nlx = ctx->p.n_l;
+    g.Nlx = ctx->p.n_l + 6;  // rows up to Nl + 5 are read by the parity products (zero)
     g.Nth = ctx->p.n_theta;
     g.h = ctx->ang.h;
     g.hq = ctx->ang.hq;
@@ -1657,8 +1660,8 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
                 (const double2 *)ctx->psi, ref, ctx->psi32, p.n_r, ctx->C, p.n_orb);
         if (pass == 0)
             k_kin_fwd_tiled<T><<<dim3((p.nk_max + TM - 1) / TM, nct, p.n_l), WG, 0, ctx->stream>>>(
-                Q, ctx->lam, ctx->nkeep, ctx->psi, ctx->psi32, ref, Y, p.n_r, p.nk_max, ctx->C,
-                p.n_orb, p.dt);
+                Q, ctx->kph, ctx->nkeep, ctx->psi, ctx->psi32, ref, Y, p.n_r, p.nk_max, ctx->C,
+                p.n_orb);
         else
             k_kin_bwd_tiled<T><<<dim3((p.n_r + TM - 1) / TM, nct, p.n_l), WG, 0, ctx->stream>>>(
                 Q, ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max, ctx->C, p.n_orb);
@@ -1667,7 +1670,7 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
     const int ncol = (2 * ctx->C + 15) / 16;
     if (pass == 0)
         k_kin_fwd<T><<<dim3(p.nk_max / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
-            Q, ctx->lam, ctx->nkeep, ctx->psi, ref, Y, p.n_r, p.nk_max, ctx->C, p.n_orb, p.dt);
+            Q, ctx->kph, ctx->nkeep, ctx->psi, ref, Y, p.n_r, p.nk_max, ctx->C, p.n_orb);
     else
         k_kin_bwd<T><<<dim3((p.n_r + 15) / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
             Q, ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max, ctx->C, p.n_orb);
@@ -1880,7 +1883,9 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
     }
     auto mark = [&](int s, int k) -> hipError_t {
         if (!ctx->timing) return hipSuccess;
-        if (ctx->timing == 1 && k != 1 && k != 2 && k != 3) return hipSuccess;  // kinetic only
+        // mode 1: kinetic passes of every TIMING_STRIDE-th step only, so that the
+        // events do not slow the timed run
+        if (ctx->timing == 1 && (k < 1 || k > 3 || s % TIMING_STRIDE)) return hipSuccess;
         return hipEventRecord(ctx->events[(size_t)s * ne + k], ctx->stream);
     };
     int s_begin = 0;
@@ -1906,7 +1911,7 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
     if (ctx->timing) {
         for (int s = 0; s < nsteps; ++s) {
             for (int k = 0; k < GGB_NPHASE; ++k) {
-                if (ctx->timing == 1 && k != 1 && k != 2) continue;
+                if (ctx->timing == 1 && (k < 1 || k > 2 || s % TIMING_STRIDE)) continue;
                 float ms = 0.f;
                 HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->events[(size_t)s * ne + k],
                                                 ctx->events[(size_t)s * ne + k + 1]));

@@ -169,9 +169,10 @@ int ggb_set_graph(ggb_ctx *ctx, int32_t steps);
 #define GGB_NPHASE 5
 
 /* Record HIP events on the context stream around the step phases of later
- * ggb_run calls (mode 1: kinetic passes only, 2: every phase, 0: off) and
- * reset the accumulated times.  No counterpart in the reference, which has
- * no timing code. */
+ * ggb_run calls (mode 1: the two kinetic passes of every 16th step, cheap
+ * enough for a timed run; 2: every phase of every step; 0: off) and reset
+ * the accumulated times.  No counterpart in the reference, which has no
+ * timing code. */
 int ggb_set_timing(ggb_ctx *ctx, int32_t mode);
 
 /* Accumulated milliseconds and launch counts per phase [GGB_NPHASE]. */
