@@ -85,6 +85,45 @@ def batched_run(sysm, pulse, args, fp64_value):
             "ratio_vs_fp64_plain": val / fp64_value}
 
 
+def serving_run(sysm_ar, args):
+    """Continuous-batching serving of a Poisson stream of scan requests (He and
+    Ne on N=256, L_max=32; two-colour Ar on the benchmark grid), FP32 mixed
+    form, arrivals in real time; p50/p99 arrival -> spectrum."""
+    from ggbtddft_amd.serving import Request, Server, poisson_arrivals
+    n = args.serving
+    rng = np.random.default_rng(0)
+    arr = poisson_arrivals(args.serving_rate, n, seed=0)
+    reqs = []
+    for k in range(n):
+        I = float(np.exp(rng.uniform(np.log(5e13), np.log(5e14))))
+        lam = float(rng.uniform(600.0, 1000.0))
+        if k % 3 == 2:
+            reqs.append(Request(rid=k, atom=ATOM, intensity=I, wavelength=lam, n_cycles=2,
+                                N=N, r_max=R_MAX, lmax=LMAX, dt=DT, arrival=float(arr[k]),
+                                second=(2, 0.1, 0.0)))
+        else:
+            reqs.append(Request(rid=k, atom="He" if k % 3 == 0 else "Ne", intensity=I,
+                                wavelength=lam, n_cycles=2, N=256, r_max=100.0, lmax=32,
+                                dt=0.05, arrival=float(arr[k])))
+    srv = Server(slots=args.serving_slots, chunk=256, ring=1 << 15, precision=1)
+    srv.systems[(ATOM, N, float(R_MAX), LMAX, float(DT))] = sysm_ar
+    for key in {r.key() for r in reqs}:   # ground states before the clock starts
+        srv._system(key)
+    results, stats = srv.serve(reqs, realtime=True)
+    srv.close()
+    bad = [r for r in results if r.error is not None or not np.all(np.isfinite(r.spectrum))]
+    stats = dict(stats)
+    stats.update({"requests": n, "arrival_rate_per_s": args.serving_rate,
+                  "slots_per_lane": args.serving_slots, "precision": "fp32 mixed",
+                  "workload": "Poisson stream: He, Ne (N=256, Lmax=32, dt=0.05) and two-colour Ar "
+                              "(N=512, Lmax=64, dt=0.02), 2-cycle pulses 5e13-5e14 W/cm^2, "
+                              "600-1000 nm",
+                  "bad_results": len(bad)})
+    log("serving: %d requests, p50 %.2f s, p99 %.2f s" % (n, stats["p50_time_to_spectrum_s"],
+                                                          stats["p99_time_to_spectrum_s"]))
+    return stats
+
+
 def measured_traffic():
     """HBM bytes per launch of the roofline kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
@@ -150,6 +189,10 @@ def main():
     ap.add_argument("--batched", type=int, default=64,
                     help="Ar instances for the batched FP32 leg (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--serving", type=int, default=24,
+                    help="requests of the serving leg (0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrivals per second")
+    ap.add_argument("--serving-slots", type=int, default=8)
     ap.add_argument("--precision-report", action="store_true",
                     help="full-length FP64 and FP32 runs; harmonic-intensity errors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -247,6 +290,8 @@ def main():
     }
     if args.batched > 0:
         line["batched_fp32"] = batched_run(sysm, pulse, args, value)
+    if args.serving > 0:
+        line["serving"] = serving_run(sysm, args)
     if args.precision_report:
         line["precision_report"] = precision_report(sysm, pulse)
     if not args.no_cpu_baseline:
