@@ -20,7 +20,7 @@ EXPORTS = ("ggb_abi_version", "ggb_create", "ggb_upload_static", "ggb_upload_orb
            "ggb_debug_substep", "ggb_set_timing", "ggb_get_timing", "ggb_selftest_mfma",
            "ggb_field_ring", "ggb_set_field_slice", "ggb_upload_columns",
            "ggb_rebuild_potential", "ggb_footprint", "ggb_selftest_copy", "ggb_set_graph",
-           "ggb_last_error", "ggb_destroy")
+           "ggb_spectrum", "ggb_last_error", "ggb_destroy")
 NPHASE = 5
 PHASES = ("half_potential_laser", "kinetic_qt", "kinetic_q", "laser_half_potential_obs",
           "potential_rebuild")
@@ -98,6 +98,10 @@ def load():
     lib.ggb_footprint.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(ctypes.c_uint64)]
     lib.ggb_selftest_copy.restype = ctypes.c_int
     lib.ggb_selftest_copy.argtypes = [ctypes.c_int32, ctypes.c_uint64, _dp]
+    lib.ggb_spectrum.restype = ctypes.c_int
+    lib.ggb_spectrum.argtypes = [ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_double,
+                                 ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+                                 _dp, ctypes.c_int32, _dp, _dp]
     lib.ggb_set_graph.restype = ctypes.c_int
     lib.ggb_set_graph.argtypes = [vp, ctypes.c_int32]
     lib.ggb_selftest_mfma.restype = ctypes.c_int

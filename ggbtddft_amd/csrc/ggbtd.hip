@@ -1139,6 +1139,157 @@ struct DvEpi {
     }
 };
 
+// ---------------------------------------------------------------------------
+// HHG spectrum on the device (analysis.py:447-460, internal.py:361-379):
+// Kaiser window (sym = False), optional 8th-order central second difference,
+// zero padding by n_pad * n on each side (M = (2 n_pad + 1) n points),
+// |fftshift(fft(fftshift(x)))|^2, and the super-Gaussian harmonic masks of
+// argonInSitu/arg.py:42-67.  M is any length: the DFT is Bluestein's
+// algorithm, X_k = w_k sum_j (y_j w_j) conj(w_{k-j}), w_j = e^{-i pi j^2 / M},
+// with the convolution done by radix-2 Stockham FFTs of length L >= 2M - 1.
+// ---------------------------------------------------------------------------
+static __device__ double bessel_i0(double x) {
+    // power series sum_k ((x/2)^2)^k / (k!)^2, all terms positive
+    const double q = 0.25 * x * x;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 200; ++k) {
+        term *= q / ((double)k * (double)k);
+        sum += term;
+        if (term < 1e-17 * sum) break;
+    }
+    return sum;
+}
+
+// x_j = signal_j * I0(beta sqrt(1 - (2j/n - 1)^2)) / I0(beta)   (scipy kaiser, sym=False)
+__global__ __launch_bounds__(WG) void k_spec_window(const double *__restrict__ sig,
+                                                    double *__restrict__ x, int n, double beta) {
+    const int j = blockIdx.x * WG + threadIdx.x;
+    if (j >= n) return;
+    const double t = 2.0 * (double)j / (double)n - 1.0;
+    x[j] = sig[j] * bessel_i0(beta * sqrt(fmax(0.0, 1.0 - t * t))) / bessel_i0(beta);
+}
+
+// 8th-order central second difference, zero on the 4 edge points at each end
+__global__ __launch_bounds__(WG) void k_spec_fd8(const double *__restrict__ x,
+                                                 double *__restrict__ y, int n, double inv_dt2) {
+    const int j = blockIdx.x * WG + threadIdx.x;
+    if (j >= n) return;
+    double v = 0.0;
+    if (n > 8 && j >= 4 && j < n - 4) {
+        const double c[5] = {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
+        v = c[0] * x[j];
+        for (int s = 1; s < 5; ++s) v += c[s] * (x[j + s] + x[j - s]);
+    }
+    y[j] = v * inv_dt2;
+}
+
+// e^{sign i pi j^2 / M}, with j^2 reduced mod 2M exactly
+static __device__ double2 chirp(long j, long M, double sign) {
+    const long r = (j * j) % (2 * M);
+    double s, c;
+    sincospi(sign * (double)r / (double)M, &s, &c);
+    return make_double2(c, s);
+}
+
+// a_j = y_j w_j for j < M (y = fftshift of the padded signal), zero up to L
+__global__ __launch_bounds__(WG) void k_blue_a(const double *__restrict__ x, double2 *__restrict__ a,
+                                               int n, int n_pad, long M, long L) {
+    const long j = (long)blockIdx.x * WG + threadIdx.x;
+    if (j >= L) return;
+    double2 v = make_double2(0.0, 0.0);
+    if (j < M) {
+        const long i = (j - M / 2 + M) % M;       // index into the padded signal
+        const long k = i - (long)n_pad * n;       // index into x
+        if (k >= 0 && k < n) {
+            const double2 w = chirp(j, M, -1.0);
+            v = make_double2(x[k] * w.x, x[k] * w.y);
+        }
+    }
+    a[j] = v;
+}
+
+// b_j = conj(w_j) for |j| < M, wrapped to length L
+__global__ __launch_bounds__(WG) void k_blue_b(double2 *__restrict__ b, long M, long L) {
+    const long j = (long)blockIdx.x * WG + threadIdx.x;
+    if (j >= L) return;
+    double2 v = make_double2(0.0, 0.0);
+    if (j < M) v = chirp(j, M, 1.0);
+    else if (j > L - M) v = chirp(L - j, M, 1.0);
+    b[j] = v;
+}
+
+// one radix-2 Stockham pass (sub-transform size Ns -> 2 Ns), natural order at the end
+__global__ __launch_bounds__(WG) void k_fft_pass(const double2 *__restrict__ in,
+                                                 double2 *__restrict__ out, long L, long Ns,
+                                                 double sign) {
+    const long j = (long)blockIdx.x * WG + threadIdx.x;
+    if (j >= L / 2) return;
+    const long k = j % Ns;
+    double s, c;
+    sincospi(sign * (double)k / (double)Ns, &s, &c);
+    const double2 v0 = in[j], u = in[j + L / 2];
+    const double2 v1 = make_double2(u.x * c - u.y * s, u.x * s + u.y * c);
+    const long d = (j / Ns) * 2 * Ns + k;
+    out[d] = make_double2(v0.x + v1.x, v0.y + v1.y);
+    out[d + Ns] = make_double2(v0.x - v1.x, v0.y - v1.y);
+}
+
+__global__ __launch_bounds__(WG) void k_cmul(double2 *__restrict__ a, const double2 *__restrict__ b,
+                                             long L) {
+    const long j = (long)blockIdx.x * WG + threadIdx.x;
+    if (j >= L) return;
+    const double2 u = a[j], v = b[j];
+    a[j] = make_double2(u.x * v.x - u.y * v.y, u.x * v.y + u.y * v.x);
+}
+
+// power[q] = |X_{(q - M/2) mod M}|^2, X_k = w_k C_k / L
+__global__ __launch_bounds__(WG) void k_blue_power(const double2 *__restrict__ C,
+                                                   double *__restrict__ power, long M, long L) {
+    const long q = (long)blockIdx.x * WG + threadIdx.x;
+    if (q >= M) return;
+    const long k = (q - M / 2 + M) % M;
+    const double2 w = chirp(k, M, -1.0), cv = C[k];
+    const double xr = (w.x * cv.x - w.y * cv.y) / (double)L, xi = (w.x * cv.y + w.y * cv.x) / (double)L;
+    power[q] = xr * xr + xi * xi;
+}
+
+// intensity of harmonic `order`: sum_q exp(-((q - i0) / (0.25 (ip - im)))^32) S_q, with
+// i0, ip, im the grid points nearest to H, H + 1/2, H - 1/2 times omega0 (one work-group
+// per order, fixed-order reduction over the band where the mask is not negligible)
+__global__ __launch_bounds__(WG) void k_harmonics(const double *__restrict__ power,
+                                                  const double *__restrict__ orders,
+                                                  double *__restrict__ out, long M, double dt,
+                                                  double omega0) {
+    __shared__ double red[WG / 64];
+    const double H = orders[blockIdx.x];
+    const double scale = (double)M * dt / (2.0 * M_PI);  // grid points per unit of omega
+    auto nearest = [&](double w) {
+        long q = (long)llround(w * scale) + M / 2;
+        return q < 0 ? 0 : (q > M - 1 ? M - 1 : q);
+    };
+    const long i0 = nearest(H * omega0), ip = nearest((H + 0.5) * omega0),
+               im = nearest((H - 0.5) * omega0);
+    const double width = 0.25 * (double)(ip - im);
+    double s = 0.0;
+    if (width > 0.0) {
+        const long half = (long)(2.0 * width) + 1;  // exp(-2^32) underflows beyond
+        const long lo = i0 - half < 0 ? 0 : i0 - half, hi = i0 + half > M - 1 ? M - 1 : i0 + half;
+        for (long q = lo + threadIdx.x; q <= hi; q += WG) {
+            const double u = ((double)q - (double)i0) / width;
+            const double u2 = u * u, u4 = u2 * u2, u8 = u4 * u4, u16 = u8 * u8;
+            s += exp(-(u16 * u16)) * power[q];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < WG / 64; ++w) t += red[w];
+        out[blockIdx.x] = t;
+    }
+}
+
 // plain streaming copy of known size (calibrates the HBM byte counters)
 __global__ __launch_bounds__(WG) void k_copy(const double2 *__restrict__ src,
                                              double2 *__restrict__ dst, size_t n) {
@@ -2017,6 +2168,93 @@ extern "C" int ggb_get_timing(const ggb_ctx *ctx, double *ms_total, int64_t *cou
         count[k] = ctx->phase_n[k];
     }
     return GGB_OK;
+}
+
+// radix-2 Stockham FFT of length L = 2^m in place of buf (tmp: scratch of L)
+static hipError_t fft_pow2(double2 *buf, double2 *tmp, long L, double sign, hipStream_t st) {
+    double2 *in = buf, *out = tmp;
+    const unsigned nb = (unsigned)((L / 2 + WG - 1) / WG);
+    for (long Ns = 1; Ns < L; Ns *= 2) {
+        k_fft_pass<<<nb, WG, 0, st>>>(in, out, L, Ns, sign);
+        double2 *t = in;
+        in = out;
+        out = t;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && in != buf) e = hipMemcpyAsync(buf, in, L * sizeof(double2), hipMemcpyDeviceToDevice, st);
+    return e;
+}
+
+extern "C" int ggb_spectrum(int32_t device, const double *signal, int32_t n, double dt,
+                            int32_t differentiate, double beta, int32_t n_pad, double omega0,
+                            const double *orders, int32_t n_orders, double *power, double *intensity) {
+    if (!signal || n < 16 || n_pad < 0 || !(dt > 0.0) || !(beta >= 0.0) || n_orders < 0 ||
+        (n_orders > 0 && (!orders || !intensity || !(omega0 > 0.0)))) {
+        g_create_err = "ggb_spectrum: bad arguments";
+        return GGB_E_ARG;
+    }
+    const long M = (2L * n_pad + 1) * n;
+    if (M > (1L << 27)) {
+        g_create_err = "ggb_spectrum: (2 n_pad + 1) n exceeds 2^27";
+        return GGB_E_ARG;
+    }
+    long L = 1;
+    while (L < 2 * M - 1) L *= 2;
+    if (hipSetDevice(device) != hipSuccess) {
+        g_create_err = "ggb_spectrum: hipSetDevice failed";
+        return GGB_E_HIP;
+    }
+    hipStream_t st = nullptr;
+    double *dsig = nullptr, *dx = nullptr, *dpow = nullptr, *dord = nullptr, *dint = nullptr;
+    double2 *A = nullptr, *B = nullptr, *T = nullptr;
+    int rc = GGB_OK;
+    auto hip = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == GGB_OK) {
+            g_create_err = std::string("ggb_spectrum: ") + what + ": " + hipGetErrorString(e);
+            rc = GGB_E_HIP;
+        }
+        return rc == GGB_OK;
+    };
+    if (hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream") &&
+        hip(dalloc(&dsig, (size_t)n), "alloc") && hip(dalloc(&dx, (size_t)n), "alloc") &&
+        hip(dalloc(&dpow, (size_t)M), "alloc") && hip(dalloc(&A, (size_t)L), "alloc") &&
+        hip(dalloc(&B, (size_t)L), "alloc") && hip(dalloc(&T, (size_t)L), "alloc") &&
+        hip(dalloc(&dord, (size_t)(n_orders > 0 ? n_orders : 1)), "alloc") &&
+        hip(dalloc(&dint, (size_t)(n_orders > 0 ? n_orders : 1)), "alloc") &&
+        hip(hipMemcpyAsync(dsig, signal, (size_t)n * 8, hipMemcpyHostToDevice, st), "upload")) {
+        const unsigned gn = (unsigned)((n + WG - 1) / WG);
+        const unsigned gL = (unsigned)((L + WG - 1) / WG);
+        const unsigned gM = (unsigned)((M + WG - 1) / WG);
+        k_spec_window<<<gn, WG, 0, st>>>(dsig, dx, n, beta);
+        double *x = dx;
+        if (differentiate) {
+            k_spec_fd8<<<gn, WG, 0, st>>>(dx, dsig, n, 1.0 / (dt * dt));
+            x = dsig;
+        }
+        k_blue_a<<<gL, WG, 0, st>>>(x, A, n, n_pad, M, L);
+        k_blue_b<<<gL, WG, 0, st>>>(B, M, L);
+        if (hip(hipGetLastError(), "launch") && hip(fft_pow2(A, T, L, -1.0, st), "fft") &&
+            hip(fft_pow2(B, T, L, -1.0, st), "fft")) {
+            k_cmul<<<gL, WG, 0, st>>>(A, B, L);
+            if (hip(fft_pow2(A, T, L, 1.0, st), "inverse fft")) {
+                k_blue_power<<<gM, WG, 0, st>>>(A, dpow, M, L);
+                if (n_orders > 0 &&
+                    hip(hipMemcpyAsync(dord, orders, (size_t)n_orders * 8, hipMemcpyHostToDevice, st), "upload"))
+                    k_harmonics<<<n_orders, WG, 0, st>>>(dpow, dord, dint, M, dt, omega0);
+                if (hip(hipGetLastError(), "launch") && power)
+                    hip(hipMemcpyAsync(power, dpow, (size_t)M * 8, hipMemcpyDeviceToHost, st), "download");
+                if (rc == GGB_OK && n_orders > 0)
+                    hip(hipMemcpyAsync(intensity, dint, (size_t)n_orders * 8, hipMemcpyDeviceToHost, st),
+                        "download");
+                hip(hipStreamSynchronize(st), "synchronize");
+            }
+        }
+    }
+    void *bufs[] = {dsig, dx, dpow, dord, dint, A, B, T};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
 }
 
 extern "C" int ggb_selftest_copy(int32_t device, uint64_t bytes, double *ms) {

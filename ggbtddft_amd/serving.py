@@ -16,8 +16,8 @@ grid, optional second colour) arrive over time and are served concurrently:
 * Each lane runs in its own host thread (ctypes releases the GIL), so work of
   different lanes overlaps on the device through their streams.
 * A request finishes when its own number of steps is done; its dipole
-  acceleration is turned into a spectrum on the host and the request's
-  arrival -> spectrum time is recorded.
+  acceleration is turned into a spectrum on the GPU (``ggb_spectrum``) and the
+  request's arrival -> spectrum time is recorded.
 
 Every instance's arithmetic is column-local in every kernel, so a request's
 result does not depend on what else shares its lane (tests check this).
@@ -154,6 +154,9 @@ class Server:
         self.lanes = {}
         self.used = 0
         self.beta = spectrum_beta
+        # spectra on the GPU with the real propagator; the host definition
+        # (same numbers, numpy FFT) with a stand-in backend
+        self.device_spectrum = factory is None
         self.lock = threading.Lock()
 
     def _system(self, key):
@@ -167,7 +170,12 @@ class Server:
         nr = np.concatenate([b[2] for b in a.buf])
         res = Result(request=a.req, dip=d, acc=ac, norm=nr)
         res.a_total = lane.system.total_acceleration(ac, nr, a.E_end)
-        res.omega, res.spectrum = spec.acceleration_spectrum(res.a_total, a.req.dt, beta=self.beta)
+        if self.device_spectrum:
+            res.omega, res.spectrum, _ = spec.device_spectrum(res.a_total, a.req.dt, beta=self.beta,
+                                                              device=self.device)
+        else:
+            res.omega, res.spectrum = spec.acceleration_spectrum(res.a_total, a.req.dt,
+                                                                 beta=self.beta)
         res.t_finish = time.perf_counter() - t0
         return res
 

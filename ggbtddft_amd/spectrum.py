@@ -11,6 +11,10 @@ dipole acceleration (no differencing).  ``harmonic_intensities`` integrates a
 spectrum under the super-Gaussian harmonic masks of
 argonInSitu/argonInSitu/arg.py:42-67; ``cutoff_energy`` is 3.17 U_p + I_p
 (internal.py:203).
+
+``device_spectrum`` computes the same power spectrum and harmonic intensities
+on the GPU (``ggb_spectrum``: window, difference, Bluestein DFT of any length
+on radix-2 Stockham FFTs, masks), for the serving path's time-to-spectrum.
 """
 import numpy as np
 from scipy.signal.windows import kaiser
@@ -60,3 +64,27 @@ def harmonic_intensities(w, S, omega0, orders):
 
 def cutoff_energy(E0, omega0, ip):
     return 3.17 * (E0 / (2.0 * omega0)) ** 2 + abs(ip)
+
+
+def device_spectrum(signal, dt, differentiate=False, beta=24.0, n_pad=8, omega0=None,
+                    orders=None, device=0, want_power=True):
+    """(w, power, intensities) of ``dipole_spectrum`` (differentiate=True) or
+    ``acceleration_spectrum`` computed on the GPU; intensities of ``orders``
+    with the masks of ``harmonic_intensities`` (None without orders)."""
+    from . import _lib
+    lib = _lib.load()
+    x = np.ascontiguousarray(signal, dtype=np.float64)
+    n = x.size
+    M = (2 * n_pad + 1) * n
+    power = np.empty(M) if want_power else None
+    ords = None if orders is None else np.ascontiguousarray(orders, dtype=np.float64)
+    inten = None if ords is None else np.empty(ords.size)
+    rc = lib.ggb_spectrum(int(device), _lib.dptr(x), n, float(dt), int(bool(differentiate)),
+                          float(beta), int(n_pad), float(omega0 or 0.0),
+                          None if ords is None else _lib.dptr(ords),
+                          0 if ords is None else ords.size,
+                          None if power is None else _lib.dptr(power),
+                          None if inten is None else _lib.dptr(inten))
+    _lib.check(rc, None, "ggb_spectrum")
+    w = 2.0 * np.pi * (np.arange(M) - M // 2) / (M * dt)
+    return w, power, inten

@@ -190,6 +190,19 @@ int ggb_selftest_copy(int32_t device, uint64_t bytes, double *ms);
 
 /* Message of the last failure on ctx (or of the last failed ggb_create when
  * ctx is NULL). */
+/* HHG spectrum of one time series on the device: Kaiser window (beta,
+ * sym = False) on signal[n], the 8th-order central second difference when
+ * `differentiate` (a dipole; else an acceleration), zero padding by
+ * n_pad * n on each side (M = (2 n_pad + 1) n), power[M] =
+ * |fftshift(fft(fftshift(.)))|^2 at omega_q = 2 pi (q - M/2) / (M dt)
+ * (power may be NULL), and intensity[k] = the harmonic-mask sum of
+ * orders[k] for fundamental omega0.  Replaces the spectrum part of
+ * analysis.py:447-460 (with internal.py:361-379) and the harmonic masks of
+ * argonInSitu/arg.py:42-67.  Errors: ggb_last_error(NULL). */
+int ggb_spectrum(int32_t device, const double *signal, int32_t n, double dt,
+                 int32_t differentiate, double beta, int32_t n_pad, double omega0,
+                 const double *orders, int32_t n_orders, double *power, double *intensity);
+
 const char *ggb_last_error(const ggb_ctx *ctx);
 
 void ggb_destroy(ggb_ctx *ctx);

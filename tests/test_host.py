@@ -118,3 +118,11 @@ def test_reference_orbitals_are_unit_eigenvectors(atom, N, R, lmax):
         assert abs(c @ c - 1.0) < 1e-10          # inside the retained space
         H = S.grid.hamiltonian(l, S.atom.Z, S.v_gs)
         assert np.linalg.norm(H @ phi - ref["eps"][k] * phi) < 1e-6 * abs(ref["eps"][k])
+
+
+def test_spectrum_rejects_bad_arguments_without_device(lib):
+    # argument checks come before any device call
+    x = np.zeros(8)
+    rc = lib.ggb_spectrum(0, _lib.dptr(x), 8, 0.05, 0, 24.0, 8, 0.0, None, 0, None, None)
+    assert rc == -1  # GGB_E_ARG
+    assert b"ggb_spectrum" in lib.ggb_last_error(None)
