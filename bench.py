@@ -71,6 +71,8 @@ def batched_run(sysm, pulse, args, fp64_value):
     E = np.stack([Pulse(I, WAVELENGTH, CYCLES).midpoints(DT)[:nst] for I in Is])
     prop = Propagator(sysm, n_inst=n, precision=1)
     prop.set_field(E)
+    tuned = prop.autotune(reps=5)
+    log("autotuner (batched fp32): %s %s" % tuned)
     prop.run(0, args.warmup)
     t0 = time.perf_counter()
     obs = prop.run(args.warmup, args.batched_steps)
@@ -82,6 +84,7 @@ def batched_run(sysm, pulse, args, fp64_value):
     return {"value": val, "unit": "orbital-timesteps/s", "instances": n,
             "steps": args.batched_steps, "ms_per_step": 1e3 * el / args.batched_steps,
             "precision": "fp32 tables/transforms/v_xc, fp64 state and sums",
+            "kinetic_variant": tuned[0], "autotuner_ms_per_kinetic_step": tuned[1],
             "ratio_vs_fp64_plain": val / fp64_value}
 
 
@@ -122,6 +125,28 @@ def serving_run(sysm_ar, args):
     log("serving: %d requests, p50 %.2f s, p99 %.2f s" % (n, stats["p50_time_to_spectrum_s"],
                                                           stats["p99_time_to_spectrum_s"]))
     return stats
+
+
+def spectrum_timing(nfull):
+    """Device vs host (numpy) spectrum of one full-length Ar acceleration
+    series (window, padding to 17 n, FFT, |.|^2, 20 harmonic masks)."""
+    from ggbtddft_amd import spectrum as spec
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(nfull)
+    orders = np.arange(1, 41, 2, dtype=np.float64)
+    w0 = 45.5633525 / WAVELENGTH
+    spec.device_spectrum(x, DT, omega0=w0, orders=orders)  # warm-up (module load, first launch)
+    t0 = time.perf_counter()
+    _, p_d, i_d = spec.device_spectrum(x, DT, omega0=w0, orders=orders)
+    t_dev = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    w_h, p_h = spec.acceleration_spectrum(x, DT)
+    i_h = spec.harmonic_intensities(w_h, p_h, w0, orders)
+    t_host = time.perf_counter() - t0
+    return {"samples": nfull, "fft_points": int(p_h.size), "device_ms": 1e3 * t_dev,
+            "host_numpy_ms": 1e3 * t_host,
+            "max_rel_diff_power": float(np.abs(p_d - p_h).max() / p_h.max()),
+            "max_rel_diff_harmonics": float(np.abs(i_d / i_h - 1).max())}
 
 
 def measured_traffic():
@@ -230,6 +255,8 @@ def main():
 
     prop = Propagator(sysm)
     prop.set_field(E)
+    tuned = prop.autotune(reps=10)
+    log("autotuner (fp64 plain): %s %s" % tuned)
     prop.run(0, args.warmup)
     log("warm-up done (%d steps)" % args.warmup)
     prop.set_timing(1)
@@ -280,6 +307,7 @@ def main():
                    "pulse": "800 nm sin^2 8 cycles 2e14 W/cm^2", "full_run_steps": nfull,
                    "precision": "fp64", "parallelism": "single GPU"},
         "phase_ms_per_step": phases,
+        "kinetic_variant": tuned[0], "autotuner_ms_per_kinetic_step": tuned[1],
         "roofline": {"kernel": "k_kin_fwd (kinetic propagator, Q^T pass)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -292,6 +320,7 @@ def main():
         line["batched_fp32"] = batched_run(sysm, pulse, args, value)
     if args.serving > 0:
         line["serving"] = serving_run(sysm, args)
+    line["spectrum"] = spectrum_timing(nfull)
     if args.precision_report:
         line["precision_report"] = precision_report(sysm, pulse)
     if not args.no_cpu_baseline:

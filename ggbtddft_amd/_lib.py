@@ -20,7 +20,9 @@ EXPORTS = ("ggb_abi_version", "ggb_create", "ggb_upload_static", "ggb_upload_orb
            "ggb_debug_substep", "ggb_set_timing", "ggb_get_timing", "ggb_selftest_mfma",
            "ggb_field_ring", "ggb_set_field_slice", "ggb_upload_columns",
            "ggb_rebuild_potential", "ggb_footprint", "ggb_selftest_copy", "ggb_set_graph",
-           "ggb_spectrum", "ggb_last_error", "ggb_destroy")
+           "ggb_spectrum", "ggb_set_kinetic_variant", "ggb_autotune", "ggb_last_error",
+           "ggb_destroy")
+KIN_VARIANTS = ("default", "narrow", "tiled16", "tiled32")
 NPHASE = 5
 PHASES = ("half_potential_laser", "kinetic_qt", "kinetic_q", "laser_half_potential_obs",
           "potential_rebuild")
@@ -98,6 +100,10 @@ def load():
     lib.ggb_footprint.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(ctypes.c_uint64)]
     lib.ggb_selftest_copy.restype = ctypes.c_int
     lib.ggb_selftest_copy.argtypes = [ctypes.c_int32, ctypes.c_uint64, _dp]
+    lib.ggb_set_kinetic_variant.restype = ctypes.c_int
+    lib.ggb_set_kinetic_variant.argtypes = [vp, ctypes.c_int32]
+    lib.ggb_autotune.restype = ctypes.c_int
+    lib.ggb_autotune.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]
     lib.ggb_spectrum.restype = ctypes.c_int
     lib.ggb_spectrum.argtypes = [ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_double,
                                  ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_double,

@@ -342,19 +342,13 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
 // Tiled kinetic passes for many columns (instances sharing the tables).
 // Work-group tile 64 (M) x 128 (real columns); wave w owns the 32 x 64
 // sub-tile (w >> 1, w & 1) as 2 x 4 MFMA tiles.  K advances in chunks of
-// kin_tk<T> (16 values) staged in LDS, double buffered, the
+// TKT values (16, or 32 in FP32) staged in LDS, double buffered, the
 // next chunk prefetched into registers while the current one is multiplied;
 // one barrier per chunk.  Rows of the LDS tiles are padded by 16 values so
 // that the four K rows a wave reads per product fall on distinct banks.
 // ---------------------------------------------------------------------------
 #define TM 64
 #define TN 128
-template <typename T> struct kin_tk;
-#ifndef KIN_TK_F32
-#define KIN_TK_F32 16  // measured: 16 beats 32 (more work-groups per CU, 28 KB of LDS each)
-#endif
-template <> struct kin_tk<float> { static constexpr int v = KIN_TK_F32; };
-template <> struct kin_tk<double> { static constexpr int v = 16; };
 #define TMP (TM + 16)
 #define TNP (TN + 16)
 
@@ -374,7 +368,7 @@ static __device__ __forceinline__ void kin_tile_step(const T (*As)[TMP], const T
     }
 }
 
-template <typename T>
+template <typename T, int TKT>
 __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
                                                       const double2 *__restrict__ kph,
                                                       const int32_t *__restrict__ nkeep,
@@ -384,7 +378,6 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
                                                       int C, int n_orb) {
     // FP32 (mixed): the state comes from psi32 (k_kin_prep), FP64: from psi
     typedef typename mfma_t<T>::v4 v4;
-    constexpr int TKT = kin_tk<T>::v;
     constexpr int AV = TKT * TM / WG;  // A values per thread (consecutive a)
     constexpr int BV = TKT * 64 / WG;  // complex B values per thread (consecutive i)
     __shared__ T As[2][TKT][TMP];      // [k = i][m = a]
@@ -483,14 +476,13 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
     }
 }
 
-template <typename T>
+template <typename T, int TKT>
 __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
                                                       const int32_t *__restrict__ nkeep,
                                                       const T *__restrict__ Y, KinRef ref,
                                                       double *__restrict__ psi, int N, int nk_max,
                                                       int C, int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
-    constexpr int TKT = kin_tk<T>::v;
     constexpr int AV = TKT * TM / WG;  // A values per thread (consecutive a of one row i)
     constexpr int BV = TKT * TN / WG;  // B values per thread (consecutive a of one column)
     __shared__ T As[2][TKT][TMP];      // [k = a][m = i]
@@ -1356,6 +1348,7 @@ struct ggb_ctx {
     double *dens, *vxc, *rho_u, *q, *V, *dv, *obs_part, *obs, *E;
     double2 *phase;
     // mixed form: ground-state reference of each orbital and its overlaps
+    int kin_variant;  // GGB_KIN_*
     bool have_ref;
     float2 *psi32;  // FP32 copy of the state for the tiled pass 1 (mixed form, C > 8)
     double *phi_ref, *eps;
@@ -1447,7 +1440,7 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     d += C * (size_t)ntiles * 3;         // obs_part
     if (p->precision) {
         d += (size_t)p->n_orb * (N + 1) + 4 * C;                 // phi_ref, eps, ab
-        if (C > KIN_NARROW_MAX_C) tbytes += Nl * C * N * 2 * 4;  // psi32
+        tbytes += Nl * C * N * 2 * 4;                          // psi32
     }
     return d * sizeof(double) + tbytes + (Nl + 2 * p->n_orb) * sizeof(int32_t);
 }
@@ -1494,6 +1487,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->nkeep = ctx->mcol = nullptr;
     ctx->dens = ctx->vxc = ctx->rho_u = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
     ctx->phase = nullptr;
+    ctx->kin_variant = GGB_KIN_DEFAULT;
     ctx->have_ref = false;
     ctx->psi32 = nullptr;
     ctx->phi_ref = ctx->eps = nullptr;
@@ -1553,7 +1547,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
         ok &= dalloc(&ctx->eps, (size_t)p->n_orb) == hipSuccess;
         ok &= dalloc(&ctx->l0, (size_t)p->n_orb) == hipSuccess;
         ok &= dalloc(&ctx->ab, C) == hipSuccess;
-        if (C > KIN_NARROW_MAX_C) ok &= dalloc(&ctx->psi32, Nl * C * N) == hipSuccess;
+        ok &= dalloc(&ctx->psi32, Nl * C * N) == hipSuccess;  // tiled pass 1 (any C, autotuner)
     }
     if (!ok) return bail("ggb_create: device allocation failed", GGB_E_NOMEM);
     // zero everything that kernels may read before it is written
@@ -1792,6 +1786,13 @@ static int launch_first_half(ggb_ctx *ctx, int step, const int *cur = nullptr) {
     return GGB_OK;
 }
 
+// kinetic kernel variant in use: the one set (ggb_set_kinetic_variant /
+// ggb_autotune) or the default (narrow up to 8 columns, else tiled, K chunks of 16)
+static int kin_variant(const ggb_ctx *ctx) {
+    if (ctx->kin_variant != GGB_KIN_DEFAULT) return ctx->kin_variant;
+    return ctx->C > KIN_NARROW_MAX_C ? GGB_KIN_TILED16 : GGB_KIN_NARROW;
+}
+
 template <typename T>
 static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
     const ggb_problem &p = ctx->p;
@@ -1804,18 +1805,29 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
     if (pass == 0 && ref.phi)
         k_kin_ref<<<ctx->C, WG, 0, ctx->stream>>>(ctx->psi, ctx->phi_ref, ctx->l0, ctx->eps, ctx->ab,
                                                   p.n_r, ctx->C, p.n_orb, p.dt);
-    if (ctx->C > KIN_NARROW_MAX_C) {
+    const int var = kin_variant(ctx);
+    if (var >= GGB_KIN_TILED16) {
         const int nct = (2 * ctx->C + TN - 1) / TN;
+        const dim3 gf((p.nk_max + TM - 1) / TM, nct, p.n_l), gb((p.n_r + TM - 1) / TM, nct, p.n_l);
         if (pass == 0 && mfma_t<T>::mixed)
             k_kin_prep<<<dim3((p.n_r + WG - 1) / WG, ctx->C, p.n_l), WG, 0, ctx->stream>>>(
                 (const double2 *)ctx->psi, ref, ctx->psi32, p.n_r, ctx->C, p.n_orb);
-        if (pass == 0)
-            k_kin_fwd_tiled<T><<<dim3((p.nk_max + TM - 1) / TM, nct, p.n_l), WG, 0, ctx->stream>>>(
-                Q, ctx->kph, ctx->nkeep, ctx->psi, ctx->psi32, ref, Y, p.n_r, p.nk_max, ctx->C,
-                p.n_orb);
+        // FP64 chunks stay at 16 (32 would need 114 KB of LDS)
+        const bool k32 = var == GGB_KIN_TILED32 && mfma_t<T>::mixed;
+        if (pass == 0 && k32)
+            k_kin_fwd_tiled<T, 32><<<gf, WG, 0, ctx->stream>>>(Q, ctx->kph, ctx->nkeep, ctx->psi,
+                                                               ctx->psi32, ref, Y, p.n_r, p.nk_max,
+                                                               ctx->C, p.n_orb);
+        else if (pass == 0)
+            k_kin_fwd_tiled<T, 16><<<gf, WG, 0, ctx->stream>>>(Q, ctx->kph, ctx->nkeep, ctx->psi,
+                                                               ctx->psi32, ref, Y, p.n_r, p.nk_max,
+                                                               ctx->C, p.n_orb);
+        else if (k32)
+            k_kin_bwd_tiled<T, 32><<<gb, WG, 0, ctx->stream>>>(Q, ctx->nkeep, Y, ref, ctx->psi,
+                                                               p.n_r, p.nk_max, ctx->C, p.n_orb);
         else
-            k_kin_bwd_tiled<T><<<dim3((p.n_r + TM - 1) / TM, nct, p.n_l), WG, 0, ctx->stream>>>(
-                Q, ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max, ctx->C, p.n_orb);
+            k_kin_bwd_tiled<T, 16><<<gb, WG, 0, ctx->stream>>>(Q, ctx->nkeep, Y, ref, ctx->psi,
+                                                               p.n_r, p.nk_max, ctx->C, p.n_orb);
         return;
     }
     const int ncol = (2 * ctx->C + 15) / 16;
@@ -2254,6 +2266,62 @@ extern "C" int ggb_spectrum(int32_t device, const double *signal, int32_t n, dou
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
+    return rc;
+}
+
+extern "C" int ggb_set_kinetic_variant(ggb_ctx *ctx, int32_t variant) {
+    if (!ctx) return GGB_E_ARG;
+    if (variant < GGB_KIN_DEFAULT || variant > GGB_KIN_TILED32 ||
+        (variant == GGB_KIN_TILED32 && ctx->p.precision == 0))
+        return fail(ctx, GGB_E_ARG, "ggb_set_kinetic_variant: unknown or unsupported variant");
+    if (variant != ctx->kin_variant) drop_graph(ctx);
+    ctx->kin_variant = variant;
+    return GGB_OK;
+}
+
+extern "C" int ggb_autotune(ggb_ctx *ctx, int32_t reps, int32_t *chosen, double *ms) {
+    if (!ctx) return GGB_E_ARG;
+    if (reps < 1 || reps > 1000) return fail(ctx, GGB_E_ARG, "ggb_autotune: reps must be 1..1000");
+    if (!ctx->have_static || !ctx->have_orbitals)
+        return fail(ctx, GGB_E_STATE, "ggb_autotune: tables and orbitals must be uploaded");
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    drop_graph(ctx);
+    const size_t bytes = (size_t)ctx->p.n_l * ctx->C * ctx->p.n_r * 2 * sizeof(double);
+    double *saved = nullptr;
+    HIPCHK(ctx, hipMalloc((void **)&saved, bytes));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    int best = GGB_KIN_DEFAULT, rc = GGB_OK;
+    const int prev = ctx->kin_variant;
+    hipError_t e = hipMemcpyAsync(saved, ctx->psi, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const int last = ctx->p.precision ? GGB_KIN_TILED32 : GGB_KIN_TILED16;
+    for (int v = GGB_KIN_NARROW; v <= last && e == hipSuccess && rc == GGB_OK; ++v) {
+        ctx->kin_variant = v;
+        rc = launch_kinetic(ctx);  // warm-up
+        if (rc == GGB_OK) e = hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps && rc == GGB_OK && e == hipSuccess; ++r) rc = launch_kinetic(ctx);
+        if (rc == GGB_OK && e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+        if (rc == GGB_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+        float el = 0.f;
+        if (rc == GGB_OK && e == hipSuccess) e = hipEventElapsedTime(&el, e0, e1);
+        if (rc == GGB_OK && e == hipSuccess) {
+            t[v] = (double)el / reps;
+            if (best == GGB_KIN_DEFAULT || t[v] < t[best]) best = v;
+        }
+        if (e == hipSuccess)  // the passes advanced the orbitals: put them back
+            e = hipMemcpyAsync(ctx->psi, saved, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(saved);
+    if (rc == GGB_OK && e != hipSuccess) rc = fail(ctx, GGB_E_HIP, std::string("ggb_autotune: ") + hipGetErrorString(e));
+    ctx->kin_variant = (rc == GGB_OK) ? best : prev;
+    if (rc == GGB_OK && chosen) *chosen = best;
+    if (rc == GGB_OK && ms)
+        for (int v = 0; v < 4; ++v) ms[v] = t[v];
     return rc;
 }
 

@@ -11,6 +11,9 @@ import numpy as np
 from . import _lib
 
 
+_TUNED = {}  # (N, n_l, nk_max, columns, precision) -> (variant, timings) of the autotuner
+
+
 def problem_for(system, n_inst=1, device=0, precision=0):
     """ggb_problem of a context holding n_inst instances of `system`."""
     p = _lib.Problem()
@@ -139,6 +142,29 @@ class Propagator:
                                      _lib.dptr(nrm)), "ggb_run")
         sh = (nsteps, self.n_inst, self.sys.n_orb)
         return dict(dip=dip.reshape(sh), acc=acc.reshape(sh), norm=nrm.reshape(sh))
+
+    def set_kinetic_variant(self, variant):
+        """Kinetic kernel variant: index or name in _lib.KIN_VARIANTS."""
+        v = _lib.KIN_VARIANTS.index(variant) if isinstance(variant, str) else int(variant)
+        self._check(self.lib.ggb_set_kinetic_variant(self.ctx, v), "ggb_set_kinetic_variant")
+
+    def autotune(self, reps=10, cache=True):
+        """Pick the fastest kinetic variant for this context's shape (timed on
+        the device, orbitals restored); remembered per shape in this process.
+        Returns (name, {name: ms per kinetic step})."""
+        key = (self.sys.N, self.sys.n_l, self.sys.nk_max, self.C, self.precision)
+        if cache and key in _TUNED:
+            name, times = _TUNED[key]
+            self.set_kinetic_variant(name)
+            return name, times
+        chosen = ctypes.c_int32()
+        ms = np.zeros(4)
+        self._check(self.lib.ggb_autotune(self.ctx, int(reps), ctypes.byref(chosen), _lib.dptr(ms)),
+                    "ggb_autotune")
+        times = {_lib.KIN_VARIANTS[v]: float(ms[v]) for v in range(1, 4) if ms[v] > 0}
+        name = _lib.KIN_VARIANTS[chosen.value]
+        _TUNED[key] = (name, times)
+        return name, times
 
     def set_graph(self, steps):
         """Replay the step loop from a hipGraph of `steps` steps (0 = off)."""

@@ -92,6 +92,8 @@ class Lane:
     def __init__(self, system, slots, chunk, ring, factory, device=0):
         self.system, self.slots, self.chunk, self.ring = system, slots, chunk, ring
         self.prop = factory(system, n_inst=slots, device=device)
+        if hasattr(self.prop, "autotune"):  # kinetic tile shapes for this lane's shape
+            self.prop.autotune(reps=3)
         self.prop.field_ring(ring)
         self.step = 0
         self.active = [None] * slots

@@ -163,6 +163,28 @@ int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step);
  * timing enabled use plain launches.  No counterpart in the reference. */
 int ggb_set_graph(ggb_ctx *ctx, int32_t steps);
 
+/* Kinetic-pass kernel variants (tile shapes).  DEFAULT: NARROW up to 8
+ * orbital columns, else TILED16.  NARROW: one 16x16 tile per work-group with
+ * the contraction split over its four waves; TILED16 / TILED32: 64x128
+ * work-group tiles, K chunks of 16 / 32 staged in LDS (TILED32 in the FP32
+ * mixed form only; FP64 uses 16). */
+#define GGB_KIN_DEFAULT 0
+#define GGB_KIN_NARROW 1
+#define GGB_KIN_TILED16 2
+#define GGB_KIN_TILED32 3
+
+/* Use kinetic variant `variant` (GGB_KIN_*) from now on.  No counterpart in
+ * the reference (one cuBLAS batched GEMM). */
+int ggb_set_kinetic_variant(ggb_ctx *ctx, int32_t variant);
+
+/* Autotuner: time both kinetic passes of every variant valid for this
+ * context (reps launches each, HIP events, on the context's own tables and
+ * orbitals, which are restored afterwards), keep the fastest, and return it
+ * in *chosen and the mean ms per variant in ms[4] (0 for variants not
+ * tried; ms may be NULL).  Needs ggb_upload_static and ggb_upload_orbitals.
+ * No counterpart in the reference. */
+int ggb_autotune(ggb_ctx *ctx, int32_t reps, int32_t *chosen, double *ms);
+
 /* Step phases timed by ggb_set_timing: 0 = half potential + laser,
  * 1 = kinetic pass Q^T, 2 = kinetic pass Q, 3 = laser + half potential +
  * observables, 4 = potential rebuild (density, v_xc, Hartree, phases). */

@@ -195,3 +195,36 @@ def test_graph_replay_matches_eager():
         for key in ("dip", "acc", "norm"):
             assert np.array_equal(a[key], b[key]), key
     assert np.array_equal(psi, psi_ref)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_kinetic_variants_agree_and_autotune(precision):
+    """Every kinetic kernel variant gives the same run (to rounding: they sum
+    K in different orders); the autotuner picks one of them and leaves the
+    orbitals as they were."""
+    _say("case: kinetic variants, Ne N=48 Lmax=6, 3 instances, precision %d" % precision)
+    dt = 0.05
+    S = AtomSystem("Ne", 48, 30.0, 6, dt)
+    nsteps = 20
+    t = (np.arange(nsteps) + 0.5) * dt
+    fields = np.stack([0.05 * np.sin(0.4 * t), -0.08 * np.cos(0.3 * t), 0.02 + 0.0 * t])
+    eng = _engine()
+    variants = ["narrow", "tiled16"] + (["tiled32"] if precision else [])
+    runs = {}
+    for v in variants:
+        with eng.Propagator(S, n_inst=3, precision=precision) as prop:
+            prop.set_field(fields)
+            prop.set_kinetic_variant(v)
+            runs[v] = (prop.run(0, nsteps), prop.download_orbitals())
+    ref_obs, ref_psi = runs["narrow"]
+    tol = 1e-12 if precision == 0 else 1e-7  # FP32 products: rounding of the small remainder
+    for v in variants[1:]:
+        obs, psi = runs[v]
+        assert np.abs(psi - ref_psi).max() < tol, v
+        assert np.abs(obs["dip"] - ref_obs["dip"]).max() < tol * max(1.0, np.abs(ref_obs["dip"]).max()), v
+    with eng.Propagator(S, n_inst=3, precision=precision) as prop:
+        prop.set_field(fields)
+        before = prop.download_orbitals()
+        name, times = prop.autotune(reps=2, cache=False)
+        assert name in variants and set(times) == set(variants)
+        assert np.array_equal(prop.download_orbitals(), before)
