@@ -277,7 +277,10 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 // (host padding, zeroed Y), so the loop runs to nkeep rounded up to 16
 // without per-element guards; rows i >= N and columns c >= C read row/column
 // 0 and are not stored.
-#define KU2 2
+#ifndef KIN_KU2
+#define KIN_KU2 2
+#endif
+#define KU2 KIN_KU2
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
                                                 const int32_t *__restrict__ nkeep,
@@ -738,12 +741,43 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     const T *Bs = GLB ? Bsrc : Tl;
     const T *Ba = Bs + (size_t)g.hq * g.sS;
 
+#ifndef ANG_PREFETCH_FP64
+#define ANG_PREFETCH_FP64 0
+#endif
+    // FP64 form: the phase factors of this lane's rows are fetched first and the
+    // table is staged in rounds of 8 loads (latency-bound grid of few work-groups)
+    constexpr bool PF = ANG_PREFETCH_FP64 && !mfma_t<T>::mixed && MODE != 2;
+    double2 fa[4], fb[4];
+    if (PF) {
+        const int ip = i0 + ilc;
+        const bool ok0 = ip < g.N && wave < mt;
+        const double2 *ph = g.phase + ((size_t)inst * g.N + (ip < g.N ? ip : 0)) * g.Nth;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = wave * 16 + mfma_t<T>::row(kr, r);
+            const bool ok = ok0 && j < g.h;
+            fa[r] = ok ? ph[j] : make_double2(1.0, 0.0);
+            fb[r] = ok ? ph[g.Nth - 1 - j] : make_double2(1.0, 0.0);
+        }
+    }
     if (!GLB) {
         // the block of one m is a multiple of 16 bytes (host pads nblk)
         const int n16 = g.nblk * (int)sizeof(T) / 16;
         const int4 *s4 = reinterpret_cast<const int4 *>(Bsrc);
         int4 *d4 = reinterpret_cast<int4 *>(Tl);
-        for (int idx = t; idx < n16; idx += nthr) d4[idx] = s4[idx];
+        if (PF) {
+            for (int base = t; base < n16; base += 8 * nthr) {
+                int4 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (base + u * nthr < n16) v[u] = s4[base + u * nthr];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (base + u * nthr < n16) d4[base + u * nthr] = v[u];
+            }
+        } else {
+            for (int idx = t; idx < n16; idx += nthr) d4[idx] = s4[idx];
+        }
     }
     // load the l-space tile
     for (int idx = t; idx < g.Nlx * RT; idx += nthr) {
@@ -787,9 +821,9 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                 if (j < g.h && iok) {
                     // mixed form: (f - 1), formed in FP64 before the rounding
                     const double one = mfma_t<T>::mixed ? 1.0 : 0.0;
-                    const double2 fa = ph[j], fb = ph[jm];
-                    const T fax = (T)(fa.x - one), fay = (T)fa.y;
-                    const T fbx = (T)(fb.x - one), fby = (T)fb.y;
+                    const double2 fa_ = PF ? fa[r] : ph[j], fb_ = PF ? fb[r] : ph[jm];
+                    const T fax = (T)(fa_.x - one), fay = (T)fa_.y;
+                    const T fbx = (T)(fb_.x - one), fby = (T)fb_.y;
                     const T ra = p ? pao : pa, ia = p ? pa : pao;
                     const T rb = p ? pbo : pb, ib = p ? pb : pbo;
                     const T qa = p ? (ra * fay + ia * fax) : (ra * fax - ia * fay);
@@ -981,17 +1015,17 @@ __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
     const int kq = (((g.K + 3) / 4 + 3) / 4) * 4;  // quarter of K, multiple of 4
     const int kb = wave * kq, ke = min(g.K, kb + kq);
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        double av[4], bv[4];
+    for (int k0 = kb; k0 < ke; k0 += 32) {  // 16 loads in flight per lane
+        double av[8], bv[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const int k = k0 + 4 * u + kr;
             const bool ok = k < ke;
             av[u] = (ok && mok) ? Ar[(long)k * g.sak] : 0.0;
             bv[u] = (ok && nok) ? Bc[(long)k * g.sbk] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc = mfma64(av[u], bv[u], acc);
+        for (int u = 0; u < 8; ++u) acc = mfma64(av[u], bv[u], acc);
     }
     if (wave > 0)
 #pragma unroll
@@ -1021,21 +1055,51 @@ struct QmomEpi {
 //   A_a = A_{a-1} (r_{a-1}/r_a)^{L+1} + q_a / r_a
 //   B_a = (r_a/r_{a+1})^L (B_{a+1} + q_{a+1} / r_{a+1}),   V = A + B
 // with the ratio powers as exp(n lr), lr[a] = log(r[a-1]/r[a]) from the host.
-// One work-group per (L, instance); each thread composes a contiguous chunk,
-// then a Hillis-Steele scan of the chunk maps in LDS.
+// One work-group per (L, instance): each thread composes the affine maps of
+// a contiguous chunk of radial nodes; the chunk maps are scanned within each
+// wave by shuffles and across the four waves through LDS (one barrier), in a
+// fixed order; each thread then re-runs its chunk from its start value.
 // ---------------------------------------------------------------------------
+// inclusive scan of affine maps x -> al x + be over the work-group, maps of
+// lower threads applied first (dir = +1) or of higher threads first (dir = -1)
+static __device__ void affine_scan(double &al, double &be, int dir, double (*sm)[2]) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const double pa = dir > 0 ? __shfl_up(al, off) : __shfl_down(al, off);
+        const double pb = dir > 0 ? __shfl_up(be, off) : __shfl_down(be, off);
+        if (dir > 0 ? lane >= off : lane + off < 64) {
+            be = pb * al + be;
+            al = pa * al;
+        }
+    }
+    if (lane == (dir > 0 ? 63 : 0)) {
+        sm[wave][0] = al;
+        sm[wave][1] = be;
+    }
+    __syncthreads();
+    // maps of the whole waves before this one, nearest last
+    double ca = 1.0, cb = 0.0;
+    for (int w = (dir > 0 ? 0 : WG / 64 - 1); dir > 0 ? w < wave : w > wave; w += dir) {
+        cb = cb * sm[w][0] + sm[w][1];
+        ca = ca * sm[w][0];
+    }
+    be = cb * al + be;
+    al = ca * al;
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
                                                 const double *__restrict__ r,
                                                 const double *__restrict__ lr,
                                                 double *__restrict__ V, int N, int n_har) {
-    __shared__ double sa[WG], sb[WG];
+    __shared__ double sm[WG / 64][2];
+    __shared__ double edge[2][WG];
     const int L = blockIdx.x, inst = blockIdx.y;
     const int t = threadIdx.x;
     const double *qL = q + ((size_t)inst * n_har + L) * N;
     double *VL = V + ((size_t)inst * n_har + L) * N;
     const int cs = (N + WG - 1) / WG;
-    const int s0 = t * cs, s1 = min(N, s0 + cs);
-
+    const int s0 = min(N, t * cs), s1 = min(N, s0 + cs);
     // ---- prefix: A_a = alpha_a A_{a-1} + beta_a
     double al = 1.0, be = 0.0;
     for (int a = s0; a < s1; ++a) {
@@ -1043,65 +1107,30 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
         al *= alpha;
         be = be * alpha + qL[a] / r[a];
     }
-    sa[t] = al;
-    sb[t] = be;
-    __syncthreads();
-    for (int off = 1; off < WG; off <<= 1) {
-        double pa = 1.0, pb = 0.0;
-        if (t >= off) {
-            pa = sa[t - off];
-            pb = sb[t - off];
+    affine_scan(al, be, 1, sm);
+    edge[0][t] = be;  // A at the end of this chunk
+    // ---- suffix: B_a = gamma_a B_{a+1} + gamma_a q_{a+1}/r_{a+1}, B_{N-1} = 0
+    double sa = 1.0, sb = 0.0;
+    for (int a = s1 - 1; a >= s0; --a) {
+        if (a == N - 1) {
+            sa = 0.0;
+            sb = 0.0;
+            continue;
         }
-        __syncthreads();
-        if (t >= off) {
-            // left map (pa, pb) then this map (sa, sb)
-            sb[t] = pb * sa[t] + sb[t];
-            sa[t] = pa * sa[t];
-        }
-        __syncthreads();
+        const double gam = exp((double)L * lr[a + 1]);
+        sa *= gam;
+        sb = sb * gam + gam * qL[a + 1] / r[a + 1];
     }
-    double A = (t > 0) ? sb[t - 1] : 0.0;
+    affine_scan(sa, sb, -1, sm);
+    edge[1][t] = sb;  // B at the start of this chunk
     __syncthreads();
+    double A = t > 0 ? edge[0][t - 1] : 0.0;
     for (int a = s0; a < s1; ++a) {
         const double alpha = (a > 0) ? exp((double)(L + 1) * lr[a]) : 0.0;
         A = A * alpha + qL[a] / r[a];
         VL[a] = A;
     }
-    __syncthreads();
-
-    // ---- suffix over reversed index: B_a = gamma_a B_{a+1} + gamma_a beta_{a+1}
-    // thread t owns the same chunk, scanned from the right
-    al = 1.0;
-    be = 0.0;
-    for (int a = s1 - 1; a >= s0; --a) {
-        if (a == N - 1) {
-            al = 0.0;  // B_{N-1} = 0
-            be = 0.0;
-            continue;
-        }
-        const double gam = exp((double)L * lr[a + 1]);
-        al *= gam;
-        be = be * gam + gam * qL[a + 1] / r[a + 1];
-    }
-    // reversed scan: thread t combines with threads t' > t
-    sa[WG - 1 - t] = al;
-    sb[WG - 1 - t] = be;
-    __syncthreads();
-    for (int off = 1; off < WG; off <<= 1) {
-        const int u = WG - 1 - t;
-        double pa = 1.0, pb = 0.0;
-        if (u >= off) {
-            pa = sa[u - off];
-            pb = sb[u - off];
-        }
-        __syncthreads();
-        if (u >= off) {
-            sb[u] = pb * sa[u] + sb[u];
-            sa[u] = pa * sa[u];
-        }
-        __syncthreads();
-    }
-    double Bv = (t < WG - 1) ? sb[WG - 1 - t - 1] : 0.0;
+    double Bv = t < WG - 1 ? edge[1][t + 1] : 0.0;
     for (int a = s1 - 1; a >= s0; --a) {
         if (a == N - 1) {
             Bv = 0.0;
@@ -1327,7 +1356,10 @@ static ang_geom ang_geometry(const ggb_problem *p) {
     a.waves = mt > nb ? mt : nb;
     const size_t tiles = (size_t)(p->n_l + 6) * 16 * sizeof(double) + (size_t)2 * a.hq * 16 * es;
     const size_t stat = (RT + 3 * ANG_MAXW) * sizeof(double);
-    a.glb = tiles + (size_t)a.nblk * es + stat > ANG_LDS_MAX;
+#ifndef ANG_FORCE_GLB
+#define ANG_FORCE_GLB 0
+#endif
+    a.glb = ANG_FORCE_GLB || tiles + (size_t)a.nblk * es + stat > ANG_LDS_MAX;
     a.lds = a.glb ? tiles : tiles + (size_t)a.nblk * es;
     return a;
 }

@@ -91,8 +91,10 @@ def test_substeps(atom, N, R, lmax):
         assert np.abs(got - ref).max() < 1e-12, np.abs(got - ref).max()
 
 
-@pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30)])
+@pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30),
+                                                   ("Kr", 64, 40.0, 8, 20), ("Xe", 64, 40.0, 10, 12)])
 def test_short_run(atom, N, R, lmax, nsteps):
+    # Kr: 13 orbitals with |m| up to 2; Xe: 4d shells, three |m| groups
     _say("case: short run", atom, N, lmax, nsteps)
     dt = 0.05
     S, O = _pair(atom, N, R, lmax, dt)
