@@ -269,7 +269,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
     }
 }
 
-// Kinetic propagator, pass 2: psi[l][c][i] = sum_a Q[l][i][a] Y[l][c][a].
+// Kinetic propagator, pass 2: psi[l][c][i] = sum_a QT[l][a][i] Y[l][c][a].
 // One work-group per 16 x 16 output tile, K = a split over the 4 waves as in
 // pass 1.  Within a wave the K order is permuted (lane kr of MFMA j in chunk u
 // takes a = k0 + 16u + 4kr + j) so that each lane reads 4 consecutive a of Q
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 #endif
 #define KU2 KIN_KU2
 template <typename T>
-__global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
+__global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ QT,
                                                 const int32_t *__restrict__ nkeep,
                                                 const T *__restrict__ Y, KinRef ref,
                                                 double *__restrict__ psi, int N, int nk_max,
@@ -300,7 +300,9 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
     const int im_ = i0 + lc;
-    const T *Qr = Q + (size_t)l * N * nk_max + (size_t)(im_ < N ? im_ : 0) * nk_max + 4 * kr;
+    // the transposed table QT[l][a][i]: for each a the 16 lanes of a row read
+    // 16 consecutive i (one 128-byte run per row, as pass 1 reads Q)
+    const T *Qc = QT + (size_t)l * nk_max * N + (im_ < N ? im_ : 0) + (size_t)4 * kr * N;
     const T *Yr = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max + 4 * kr;
     const int kq = (((kend + 15) / 16 + 3) / 4) * 16;  // quarter of K, multiple of 16
     const int kb = wave * kq, ke = min(kend, kb + kq);
@@ -310,7 +312,8 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ Q,
 #pragma unroll
         for (int u = 0; u < KU2; ++u) {
             if (k0 + 16 * u < ke) {
-                av[u] = *reinterpret_cast<const v4 *>(Qr + k0 + 16 * u);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) av[u][j] = Qc[(size_t)(k0 + 16 * u + j) * N];
                 bv[u] = *reinterpret_cast<const v4 *>(Yr + k0 + 16 * u);
             } else {
                 av[u] = (v4){0, 0, 0, 0};
@@ -1374,7 +1377,7 @@ struct ggb_ctx {
     int n_field;
     bool field_ring;  // ggb_field_ring: field index = step mod n_field
     // device buffers
-    double *psi, *Y, *Q, *B, *PL, *PLT, *chi, *omega, *r, *lr, *wr, *v_gs, *mask, *occ, *rho_frozen;
+    double *psi, *Y, *Q, *QT, *B, *PL, *PLT, *chi, *omega, *r, *lr, *wr, *v_gs, *mask, *occ, *rho_frozen;
     int32_t *nkeep, *mcol;
     double2 *kph;  // [n_l][nk_max] (cos, sin)(lam dt) of the kinetic propagator
     double *dens, *vxc, *rho_u, *q, *V, *dv, *obs_part, *obs, *E;
@@ -1440,7 +1443,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->p.device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->kph, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
+    void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->QT, ctx->kph, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
                     ctx->r, ctx->lr, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
@@ -1461,7 +1464,7 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     d += Nl * C * N * 2;                 // psi
     const size_t es = p->precision ? 4 : 8;
     const size_t nblk = ang_geometry(p).nblk;
-    size_t tbytes = (Nl * C * nk * 2 + Nl * N * nk + (size_t)p->n_m * nblk) * es;  // Y, Q, B
+    size_t tbytes = (Nl * C * nk * 2 + 2 * Nl * N * nk + (size_t)p->n_m * nblk) * es;  // Y, Q, QT, B
     d += 2 * Nl * nk;                    // e^{-i lam dt}
     d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
     d += 6 * N + p->n_orb;               // r, lr, wr, v_gs, mask, rho_frozen, occ
@@ -1514,7 +1517,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->field_ring = false;
     ctx->obs_cap = 0;
     ctx->kph = nullptr;
-    ctx->psi = ctx->Y = ctx->Q = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
+    ctx->psi = ctx->Y = ctx->Q = ctx->QT = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
     ctx->r = ctx->lr = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
     ctx->nkeep = ctx->mcol = nullptr;
     ctx->dens = ctx->vxc = ctx->rho_u = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
@@ -1552,6 +1555,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     const size_t nB = (size_t)p->n_m * ctx->ang.nblk;
     ok &= dalloc(&ctx->Y, (nY * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->Q, (nQ * es + 7) / 8) == hipSuccess;
+    ok &= dalloc(&ctx->QT, (nQ * es + 7) / 8) == hipSuccess;  // [l][a][i] for pass 2
     ok &= dalloc(&ctx->kph, Nl * (size_t)p->nk_max) == hipSuccess;
     ok &= dalloc(&ctx->nkeep, Nl) == hipSuccess;
     ok &= dalloc(&ctx->B, (nB * es + 7) / 8) == hipSuccess;
@@ -1584,6 +1588,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     if (!ok) return bail("ggb_create: device allocation failed", GGB_E_NOMEM);
     // zero everything that kernels may read before it is written
     if (hipMemset(ctx->B, 0, nB * es) != hipSuccess || hipMemset(ctx->Q, 0, nQ * es) != hipSuccess ||
+        hipMemset(ctx->QT, 0, nQ * es) != hipSuccess ||
         hipMemset(ctx->kph, 0, Nl * (size_t)p->nk_max * sizeof(double2)) != hipSuccess ||
         hipMemset(ctx->Y, 0, nY * es) != hipSuccess ||
         hipMemset(ctx->psi, 0, Nl * C * N * 2 * sizeof(double)) != hipSuccess ||
@@ -1622,12 +1627,25 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->wr, s->wr, N * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->v_gs, s->v_gs, N * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->mask, s->mask, N * 8, hipMemcpyHostToDevice, st));
-    if (p.precision == 0) {
-        HIPCHK(ctx, hipMemcpyAsync(ctx->Q, s->Q, Nl * N * p.nk_max * 8, hipMemcpyHostToDevice, st));
-    } else {
-        std::vector<float> q32(Nl * N * p.nk_max);
-        for (size_t k = 0; k < q32.size(); ++k) q32[k] = (float)s->Q[k];
-        HIPCHK(ctx, hipMemcpy(ctx->Q, q32.data(), q32.size() * 4, hipMemcpyHostToDevice));
+    {
+        // Q[l][i][a] and its transpose QT[l][a][i] (read by pass 2), compute type
+        const size_t nk = p.nk_max, nQ = Nl * N * nk;
+        if (p.precision == 0) {
+            HIPCHK(ctx, hipMemcpy(ctx->Q, s->Q, nQ * 8, hipMemcpyHostToDevice));
+            std::vector<double> qt(nQ);
+            for (size_t l = 0; l < Nl; ++l)
+                for (size_t i = 0; i < N; ++i)
+                    for (size_t a = 0; a < nk; ++a) qt[(l * nk + a) * N + i] = s->Q[(l * N + i) * nk + a];
+            HIPCHK(ctx, hipMemcpy(ctx->QT, qt.data(), nQ * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> q32(nQ), qt(nQ);
+            for (size_t k = 0; k < nQ; ++k) q32[k] = (float)s->Q[k];
+            for (size_t l = 0; l < Nl; ++l)
+                for (size_t i = 0; i < N; ++i)
+                    for (size_t a = 0; a < nk; ++a) qt[(l * nk + a) * N + i] = q32[(l * N + i) * nk + a];
+            HIPCHK(ctx, hipMemcpy(ctx->Q, q32.data(), nQ * 4, hipMemcpyHostToDevice));
+            HIPCHK(ctx, hipMemcpy(ctx->QT, qt.data(), nQ * 4, hipMemcpyHostToDevice));
+        }
     }
     {
         // e^{-i lam dt} of every retained eigenvalue, in FP64 on the host
@@ -1868,7 +1886,8 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
             Q, ctx->kph, ctx->nkeep, ctx->psi, ref, Y, p.n_r, p.nk_max, ctx->C, p.n_orb);
     else
         k_kin_bwd<T><<<dim3((p.n_r + 15) / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
-            Q, ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max, ctx->C, p.n_orb);
+            reinterpret_cast<const T *>(ctx->QT), ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max,
+            ctx->C, p.n_orb);
 }
 
 static int launch_kinetic_pass(ggb_ctx *ctx, int pass) {
