@@ -161,15 +161,18 @@ __global__ __launch_bounds__(WG) void k_kin_ref(const double *__restrict__ psi,
 
 // FP32 copy of the state for the tiled pass 1, the ground-state component
 // alpha phi taken out (mixed form): psi32[l][c][i] = psi - [l = l0] alpha phi.
+// only_l0: just the ground-state block of each column (the other blocks were
+// written by the angular kernel's first half, which ends every step's input)
 __global__ __launch_bounds__(WG) void k_kin_prep(const double2 *__restrict__ psi, KinRef ref,
                                                  float2 *__restrict__ psi32, int N, int C,
-                                                 int n_orb) {
+                                                 int n_orb, int only_l0) {
     const int i = blockIdx.x * WG + threadIdx.x;
-    const int c = blockIdx.y, l = blockIdx.z;
+    const int c = blockIdx.y;
+    const int k = c % n_orb;
+    const int l = only_l0 ? ref.l0[k] : blockIdx.z;
     if (i >= N) return;
     const size_t o = ((size_t)l * C + c) * N + i;
     double2 v = psi[o];
-    const int k = c % n_orb;
     if (ref.phi && ref.l0[k] == l) {
         const double4 a = ref.ab[c];
         const double f = ref.phi[(size_t)k * N + i];
@@ -605,6 +608,8 @@ struct AngArgs {
     double *dens;          // [C][N][Nth] |Psi|^2
     double *obs_part;      // [C][ntiles][3]
     const int *cur;        // graph replay: field step = (cur[0] + step) mod n_field
+    float2 *psi32;         // mode 0, FP32 tiled kinetic: FP32 copy of the stored state
+    const int32_t *l0;     // blocks left to k_kin_prep (ground-state component), or null
     int N, Nl, Nlx, Nth, h, hq, sS, sA, nblk, C, n_orb, n_field, step;
     double dt;
 };
@@ -853,12 +858,16 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     }
 
     if (MODE != 2) {
+        const int lref = (MODE == 0 && g.psi32 && g.l0) ? g.l0[korb] : -1;
         for (int idx = t; idx < g.Nl * RT; idx += nthr) {
             const int l = idx / RT, il = idx % RT;
             const int ii = i0 + il;
-            if (ii < g.N)
-                ((double2 *)g.psi)[((size_t)l * g.C + c) * g.N + ii] =
-                    make_double2((double)X[l * 16 + 2 * il], (double)X[l * 16 + 2 * il + 1]);
+            if (ii < g.N) {
+                const size_t o = ((size_t)l * g.C + c) * g.N + ii;
+                const double xr = X[l * 16 + 2 * il], xi = X[l * 16 + 2 * il + 1];
+                ((double2 *)g.psi)[o] = make_double2(xr, xi);
+                if (MODE == 0 && g.psi32 && l != lref) g.psi32[o] = make_float2((float)xr, (float)xi);
+            }
         }
     }
     if (MODE == 0) return;
@@ -1760,6 +1769,8 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
     g.step = step;
     g.dt = ctx->p.dt;
     g.cur = nullptr;
+    g.psi32 = nullptr;
+    g.l0 = nullptr;
     return g;
 }
 
@@ -1773,10 +1784,16 @@ static void launch_angular_t(ggb_ctx *ctx, const AngArgs &g) {
         k_angular<MODE, T, false><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
 }
 
+static int kin_variant(const ggb_ctx *ctx);
+
 template <int MODE>
 static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
     AngArgs g = ang_args(ctx, step);
     g.cur = cur;
+    if (MODE == 0 && ctx->p.precision && kin_variant(ctx) >= GGB_KIN_TILED16) {
+        g.psi32 = ctx->psi32;  // the kinetic pass 1 that follows reads it
+        g.l0 = ctx->have_ref ? ctx->l0 : nullptr;
+    }
     if (ctx->p.precision)
         launch_angular_t<MODE, float>(ctx, g);
     else
@@ -1843,8 +1860,10 @@ static int kin_variant(const ggb_ctx *ctx) {
     return ctx->C > KIN_NARROW_MAX_C ? GGB_KIN_TILED16 : GGB_KIN_NARROW;
 }
 
+// fresh: the angular first half has just written psi32 (all blocks but the
+// ground-state ones); otherwise pass 1 converts every block itself
 template <typename T>
-static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
+static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh) {
     const ggb_problem &p = ctx->p;
     const T *Q = reinterpret_cast<const T *>(ctx->Q);
     T *Y = reinterpret_cast<T *>(ctx->Y);
@@ -1859,9 +1878,9 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
     if (var >= GGB_KIN_TILED16) {
         const int nct = (2 * ctx->C + TN - 1) / TN;
         const dim3 gf((p.nk_max + TM - 1) / TM, nct, p.n_l), gb((p.n_r + TM - 1) / TM, nct, p.n_l);
-        if (pass == 0 && mfma_t<T>::mixed)
-            k_kin_prep<<<dim3((p.n_r + WG - 1) / WG, ctx->C, p.n_l), WG, 0, ctx->stream>>>(
-                (const double2 *)ctx->psi, ref, ctx->psi32, p.n_r, ctx->C, p.n_orb);
+        if (pass == 0 && mfma_t<T>::mixed && !(fresh && !ref.phi))
+            k_kin_prep<<<dim3((p.n_r + WG - 1) / WG, ctx->C, fresh ? 1 : p.n_l), WG, 0, ctx->stream>>>(
+                (const double2 *)ctx->psi, ref, ctx->psi32, p.n_r, ctx->C, p.n_orb, fresh ? 1 : 0);
         // FP64 chunks stay at 16 (32 would need 114 KB of LDS)
         const bool k32 = var == GGB_KIN_TILED32 && mfma_t<T>::mixed;
         if (pass == 0 && k32)
@@ -1890,18 +1909,18 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass) {
             ctx->C, p.n_orb);
 }
 
-static int launch_kinetic_pass(ggb_ctx *ctx, int pass) {
+static int launch_kinetic_pass(ggb_ctx *ctx, int pass, bool fresh = false) {
     if (ctx->p.precision)
-        launch_kinetic_t<float>(ctx, pass);
+        launch_kinetic_t<float>(ctx, pass, fresh);
     else
-        launch_kinetic_t<double>(ctx, pass);
+        launch_kinetic_t<double>(ctx, pass, fresh);
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
 }
 
-static int launch_kinetic(ggb_ctx *ctx) {
-    int rc = launch_kinetic_pass(ctx, 0);
-    return rc ? rc : launch_kinetic_pass(ctx, 1);
+static int launch_kinetic(ggb_ctx *ctx, bool fresh = false) {
+    int rc = launch_kinetic_pass(ctx, 0, fresh);
+    return rc ? rc : launch_kinetic_pass(ctx, 1, fresh);
 }
 
 static hipError_t no_mark() { return hipSuccess; }
@@ -2036,7 +2055,7 @@ static int run_graph(ggb_ctx *ctx, int step0, int nsteps, int *done) {
         int rc = GGB_OK;
         for (int s = 0; s < G && !rc; ++s) {
             rc = launch_first_half(ctx, s, ctx->dcur);
-            if (!rc) rc = launch_kinetic(ctx);
+            if (!rc) rc = launch_kinetic(ctx, true);
             if (!rc) rc = launch_second_half(ctx, s, ctx->obs, ctx->dcur);
         }
         if (!rc) k_bump<<<1, 64, 0, ctx->stream>>>(ctx->dcur, G);
@@ -2112,9 +2131,9 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         HIPCHK(ctx, mark(s, 0));
         int rc = launch_first_half(ctx, step);
         HIPCHK(ctx, mark(s, 1));
-        if (!rc) rc = launch_kinetic_pass(ctx, 0);
+        if (!rc) rc = launch_kinetic_pass(ctx, 0, true);
         HIPCHK(ctx, mark(s, 2));
-        if (!rc) rc = launch_kinetic_pass(ctx, 1);
+        if (!rc) rc = launch_kinetic_pass(ctx, 1, true);
         HIPCHK(ctx, mark(s, 3));
         if (!rc) rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3, nullptr,
                                          [&]() { return mark(s, 4); });
