@@ -605,7 +605,7 @@ struct AngArgs {
     const double *r;
     const double *chi;     // [Nth]
     const double *E;       // [n_inst][n_field]
-    double *dens;          // [C][N][Nth] |Psi|^2
+    void *dens;            // [C][N][Nth] |Psi|^2 in the compute type
     double *obs_part;      // [C][ntiles][3]
     const int *cur;        // graph replay: field step = (cur[0] + step) mod n_field
     float2 *psi32;         // mode 0, FP32 tiled kinetic: FP32 copy of the stored state
@@ -878,7 +878,7 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     if (wave < mt) {
         v4 e, o;
         ang_fwd<T>(Bs, Ba, X, g, pr, m, wave, e, o);
-        double *dens = g.dens + ((size_t)c * g.N + (iok ? i : 0)) * g.Nth;
+        T *dens = reinterpret_cast<T *>(g.dens) + ((size_t)c * g.N + (iok ? i : 0)) * g.Nth;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int j = wave * 16 + mfma_t<T>::row(kr, r);
@@ -889,10 +889,10 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
             const double s2b = vb * vb + __shfl_xor(vb * vb, 1);
             const double eo = 4.0 * (ed * od + __shfl_xor(ed * od, 1));  // |Psi_j|^2 - |Psi_j'|^2
             if (p == 0 && iok && j < g.h) {
-                dens[j] = s2a;
+                dens[j] = (T)s2a;
                 if (jm != j) {
                     const double ca = g.chi[j];
-                    dens[jm] = s2b;
+                    dens[jm] = (T)s2b;
                     sd += ri * ca * eo;
                     sa += ca * eo / (ri * ri);
                     sn += s2a + s2b;
@@ -982,7 +982,7 @@ static __device__ T lda_vxc(T rho) {
 // rho_u(i, j) = sum_k occ_k |Psi_k(i, j)|^2 (+ the frozen core), kept for the
 // multipole moments, and v_xc of the density rho = rho_u / (2 pi omega_j wr_i r_i^2).
 template <typename T>
-__global__ __launch_bounds__(WG) void k_rho(const double *__restrict__ dens,
+__global__ __launch_bounds__(WG) void k_rho(const T *__restrict__ dens,
                                             const double *__restrict__ occ,
                                             const double *__restrict__ rho_frozen,
                                             const double *__restrict__ r,
@@ -1808,7 +1808,7 @@ static int launch_potential(ggb_ctx *ctx) {
     if (p.interacting) {
         const dim3 gr((unsigned)(((size_t)N * Nth + WG - 1) / WG), p.n_inst);
         if (p.precision)
-            k_rho<float><<<gr, WG, 0, st>>>(ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
+            k_rho<float><<<gr, WG, 0, st>>>(reinterpret_cast<const float *>(ctx->dens), ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
                                             ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb);
         else
             k_rho<double><<<gr, WG, 0, st>>>(ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
