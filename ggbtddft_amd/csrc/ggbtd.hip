@@ -922,23 +922,27 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
 }
 
 // per-column sums of the observable partials -> obs[step_rel][c][3]
-__global__ __launch_bounds__(64) void k_obs_reduce(const double *__restrict__ obs_part,
-                                                   double *__restrict__ obs, int C, int ntiles,
-                                                   const int *cur, const int *origin, int local) {
-    // one wave per (column, observable); lane t sums tiles t, t+64, ... then a
-    // fixed shuffle tree, so the result does not depend on timing
-    const int idx = blockIdx.x;
+struct ObsArgs {
+    const double *part;  // [C][ntiles][3]
+    double *obs;         // destination rows, or null (no reduction)
+    int C, ntiles;
+    const int *cur, *origin;  // graph replay: row = cur[0] - origin[0] + local
+    int local;
+};
+// one wave per (column, observable) pair idx; lane t sums tiles t, t+64, ...
+// then a fixed shuffle tree, so the result does not depend on timing
+static __device__ void obs_reduce_pair(const ObsArgs &o, int idx) {
     const int c = idx / 3, k = idx % 3;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     double s = 0.0;
-    for (int tl = lane; tl < ntiles; tl += 64) s += obs_part[((size_t)c * ntiles + tl) * 3 + k];
+    for (int tl = lane; tl < o.ntiles; tl += 64) s += o.part[((size_t)c * o.ntiles + tl) * 3 + k];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
     if (lane == 0) {
-        // graph replay: the destination row is the step's index within the run call
-        const size_t row = cur ? (size_t)(cur[0] - origin[0] + local) : 0;
-        obs[row * C * 3 + idx] = s;
+        const size_t row = o.cur ? (size_t)(o.cur[0] - o.origin[0] + o.local) : 0;
+        o.obs[row * o.C * 3 + idx] = s;
     }
 }
+__global__ __launch_bounds__(64) void k_obs_reduce(ObsArgs o) { obs_reduce_pair(o, blockIdx.x); }
 
 // advances the replayed graph's step counter
 __global__ void k_bump(int *cur, int by) {
@@ -989,7 +993,15 @@ __global__ __launch_bounds__(WG) void k_rho(const T *__restrict__ dens,
                                             const double *__restrict__ wr,
                                             const double *__restrict__ omega,
                                             double *__restrict__ rho_u, double *__restrict__ vxc,
-                                            int N, int Nth, int n_orb) {
+                                            int N, int Nth, int n_orb, ObsArgs ob) {
+    // the blocks past the (r, theta) grid (instance row 0) reduce the observable
+    // partials of the step, one (column, observable) pair per wave
+    const int nrho = (int)(((size_t)N * Nth + WG - 1) / WG);
+    if ((int)blockIdx.x >= nrho) {
+        const int pair = ((int)blockIdx.x - nrho) * (WG / 64) + (threadIdx.x >> 6);
+        if (blockIdx.y == 0 && ob.obs && pair < ob.C * 3) obs_reduce_pair(ob, pair);
+        return;
+    }
     const int inst = blockIdx.y;
     const size_t idx = (size_t)blockIdx.x * WG + threadIdx.x;
     if (idx >= (size_t)N * Nth) return;
@@ -1801,18 +1813,26 @@ static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
 }
 
 // potential rebuild from the |Psi|^2 tiles already in ctx->dens
-static int launch_potential(ggb_ctx *ctx) {
+// ob: the observable reduction of the step, folded into k_rho (interacting) or
+// launched on its own
+static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
     const ggb_problem &p = ctx->p;
     hipStream_t st = ctx->stream;
     const int N = p.n_r, Nth = p.n_theta, nh = p.n_har;
+    ObsArgs none;
+    none.obs = nullptr;
+    const ObsArgs oa = ob ? *ob : none;
+    if (oa.obs && !p.interacting) k_obs_reduce<<<oa.C * 3, 64, 0, st>>>(oa);
     if (p.interacting) {
-        const dim3 gr((unsigned)(((size_t)N * Nth + WG - 1) / WG), p.n_inst);
+        const unsigned nrho = (unsigned)(((size_t)N * Nth + WG - 1) / WG);
+        const unsigned nobs = oa.obs ? (unsigned)((oa.C * 3 + WG / 64 - 1) / (WG / 64)) : 0u;
+        const dim3 gr(nrho + nobs, p.n_inst);
         if (p.precision)
             k_rho<float><<<gr, WG, 0, st>>>(reinterpret_cast<const float *>(ctx->dens), ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
-                                            ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb);
+                                            ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb, oa);
         else
             k_rho<double><<<gr, WG, 0, st>>>(ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
-                                             ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb);
+                                             ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb, oa);
         GemmOperands gq;  // q[L][i] = sum_j PLT[L][j] rho_u[i][j]
         gq.A = ctx->PLT;
         gq.sam = Nth;
@@ -1929,12 +1949,17 @@ template <typename Mark = hipError_t (*)()>
 static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int *cur = nullptr,
                               Mark between = no_mark) {
     launch_angular<1>(ctx, step, cur);
-    if (obs_dst)
-        k_obs_reduce<<<ctx->C * 3, 64, 0, ctx->stream>>>(
-            ctx->obs_part, obs_dst, ctx->C, ctx->ntiles, cur, cur ? cur + 1 : nullptr, step);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, between());  // timing event between the angular half and the rebuild
-    return launch_potential(ctx);
+    ObsArgs ob;
+    ob.part = ctx->obs_part;
+    ob.obs = obs_dst;
+    ob.C = ctx->C;
+    ob.ntiles = ctx->ntiles;
+    ob.cur = cur;
+    ob.origin = cur ? cur + 1 : nullptr;
+    ob.local = step;
+    return launch_potential(ctx, &ob);
 }
 
 extern "C" int ggb_upload_orbitals(ggb_ctx *ctx, const double *psi) {
