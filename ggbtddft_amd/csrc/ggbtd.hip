@@ -1028,8 +1028,9 @@ struct GemmOperands {
 template <class Epi>
 __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
     __shared__ double red[3][64][4];
-    const int z = blockIdx.z;
-    const int m0 = blockIdx.y * 16, n0 = blockIdx.x * 16;
+    const tile3 tb = xcd_tile();  // n tiles of one m tile run on one XCD and share A in its L2
+    const int z = tb.z;
+    const int m0 = tb.y * 16, n0 = tb.x * 16;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int m = m0 + lc, n = n0 + lc;
