@@ -601,7 +601,8 @@ struct AngArgs {
     double *psi;
     const void *Bh;        // [n_m][nblk] parity half tables, compute type
     const int32_t *mcol;   // [n_orb]
-    const double2 *phase;  // [n_inst][N][Nth] mask e^{-i dV dt/2}
+    const void *phase;     // [n_inst][N][Nth] f = mask e^{-i dV dt/2}: double2 f (FP64 form),
+                           // float2 (f - 1) (mixed form)
     const double *r;
     const double *chi;     // [Nth]
     const double *E;       // [n_inst][n_field]
@@ -749,43 +750,15 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     const T *Bs = GLB ? Bsrc : Tl;
     const T *Ba = Bs + (size_t)g.hq * g.sS;
 
-#ifndef ANG_PREFETCH_FP64
-#define ANG_PREFETCH_FP64 0
-#endif
-    // FP64 form: the phase factors of this lane's rows are fetched first and the
-    // table is staged in rounds of 8 loads (latency-bound grid of few work-groups)
-    constexpr bool PF = ANG_PREFETCH_FP64 && !mfma_t<T>::mixed && MODE != 2;
-    double2 fa[4], fb[4];
-    if (PF) {
-        const int ip = i0 + ilc;
-        const bool ok0 = ip < g.N && wave < mt;
-        const double2 *ph = g.phase + ((size_t)inst * g.N + (ip < g.N ? ip : 0)) * g.Nth;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int j = wave * 16 + mfma_t<T>::row(kr, r);
-            const bool ok = ok0 && j < g.h;
-            fa[r] = ok ? ph[j] : make_double2(1.0, 0.0);
-            fb[r] = ok ? ph[g.Nth - 1 - j] : make_double2(1.0, 0.0);
-        }
-    }
+    const int i = i0 + ilc;
+    const bool iok = i < g.N;
+    const size_t pbase = ((size_t)inst * g.N + (iok ? i : 0)) * g.Nth;
     if (!GLB) {
         // the block of one m is a multiple of 16 bytes (host pads nblk)
         const int n16 = g.nblk * (int)sizeof(T) / 16;
         const int4 *s4 = reinterpret_cast<const int4 *>(Bsrc);
         int4 *d4 = reinterpret_cast<int4 *>(Tl);
-        if (PF) {
-            for (int base = t; base < n16; base += 8 * nthr) {
-                int4 v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (base + u * nthr < n16) v[u] = s4[base + u * nthr];
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (base + u * nthr < n16) d4[base + u * nthr] = v[u];
-            }
-        } else {
-            for (int idx = t; idx < n16; idx += nthr) d4[idx] = s4[idx];
-        }
+        for (int idx = t; idx < n16; idx += nthr) d4[idx] = s4[idx];
     }
     // load the l-space tile
     for (int idx = t; idx < g.Nlx * RT; idx += nthr) {
@@ -811,14 +784,11 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         __syncthreads();
     }
 
-    const int i = i0 + ilc;
-    const bool iok = i < g.N;
     if (MODE != 2) {
         // Vh: theta space, phase, folded sums S/D, back to l space
         if (wave < mt) {
             v4 e, o;
             ang_fwd<T>(Bs, Ba, X, g, pr, m, wave, e, o);
-            const double2 *ph = g.phase + ((size_t)inst * g.N + (iok ? i : 0)) * g.Nth;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int j = wave * 16 + mfma_t<T>::row(kr, r);
@@ -827,11 +797,17 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                 const T pao = __shfl_xor(pa, 1), pbo = __shfl_xor(pb, 1);
                 T sv = (T)0, dv = (T)0;
                 if (j < g.h && iok) {
-                    // mixed form: (f - 1), formed in FP64 before the rounding
-                    const double one = mfma_t<T>::mixed ? 1.0 : 0.0;
-                    const double2 fa_ = PF ? fa[r] : ph[j], fb_ = PF ? fb[r] : ph[jm];
-                    const T fax = (T)(fa_.x - one), fay = (T)fa_.y;
-                    const T fbx = (T)(fb_.x - one), fby = (T)fb_.y;
+                    T fax, fay, fbx, fby;
+                    if (mfma_t<T>::mixed) {
+                        // mixed form: the table holds (f - 1), formed in FP64 before the rounding
+                        const float2 *ph = reinterpret_cast<const float2 *>(g.phase) + pbase;
+                        const float2 fa_ = ph[j], fb_ = ph[jm];
+                        fax = fa_.x, fay = fa_.y, fbx = fb_.x, fby = fb_.y;
+                    } else {
+                        const double2 *ph = reinterpret_cast<const double2 *>(g.phase) + pbase;
+                        const double2 fa_ = ph[j], fb_ = ph[jm];
+                        fax = (T)fa_.x, fay = (T)fa_.y, fbx = (T)fb_.x, fby = (T)fb_.y;
+                    }
                     const T ra = p ? pao : pa, ia = p ? pa : pao;
                     const T rb = p ? pbo : pb, ib = p ? pb : pbo;
                     const T qa = p ? (ra * fay + ia * fax) : (ra * fax - ia * fay);
@@ -1169,11 +1145,14 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
 
 // dV = sum_L V_L(r_i) P_L(chi_j) + v_xc - v_gs and phase = mask e^{-i dV dt/2}:
 // the Legendre sum as a tile product (M = i, N = j, K = L), this epilogue.
+// The mixed form stores float2 (f - 1): the angular kernel applies the
+// potential half step as X += B^T ((f - 1) B X) and only needs that difference,
+// formed here in FP64 and rounded once.
 struct DvEpi {
     const double *vxc, *v_gs, *mask;
     double *dv;
-    double2 *phase;
-    int N, Nth, interacting;
+    void *phase;
+    int N, Nth, interacting, mixed;
     double dt;
     __device__ void operator()(int z, int i, int j, double v) const {
         const size_t o = ((size_t)z * N + i) * Nth + j;
@@ -1181,7 +1160,11 @@ struct DvEpi {
         dv[o] = sv;
         double sn, cs;
         sincos(0.5 * dt * sv, &sn, &cs);
-        phase[o] = make_double2(mask[i] * cs, -mask[i] * sn);
+        const double fr = mask[i] * cs, fi = -mask[i] * sn;
+        if (mixed)
+            reinterpret_cast<float2 *>(phase)[o] = make_float2((float)(fr - 1.0), (float)fi);
+        else
+            reinterpret_cast<double2 *>(phase)[o] = make_double2(fr, fi);
     }
 };
 
@@ -1862,7 +1845,7 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
     gv.M = N;
     gv.N = Nth;
     gv.K = p.interacting ? nh : 0;
-    DvEpi ev{ctx->vxc, ctx->v_gs, ctx->mask, ctx->dv, ctx->phase, N, Nth, p.interacting, p.dt};
+    DvEpi ev{ctx->vxc, ctx->v_gs, ctx->mask, ctx->dv, ctx->phase, N, Nth, p.interacting, p.precision, p.dt};
     k_tile_gemm<<<dim3((Nth + 15) / 16, (N + 15) / 16, p.n_inst), WG, 0, st>>>(gv, ev);
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
