@@ -149,6 +149,31 @@ def spectrum_timing(nfull):
             "max_rel_diff_harmonics": float(np.abs(i_d / i_h - 1).max())}
 
 
+def ground_state_timing():
+    """LDA ground state of the benchmark grid on the device (ggb_ground_state)
+    and by the host iteration (numpy, one dense eigendecomposition per l and
+    iteration); returns the stats and the device ground state."""
+    from ggbtddft_amd.atoms import Atom
+    from ggbtddft_amd.grid import RadialGrid
+    from ggbtddft_amd.groundstate import DeviceGroundState, GroundState
+    DeviceGroundState(RadialGrid(48, 30.0), Atom("He"))  # warm-up (library load, first launches)
+    grid, atom = RadialGrid(N, R_MAX), Atom(ATOM)
+    t0 = time.perf_counter()
+    dev = DeviceGroundState(grid, atom)
+    t_dev = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    host = GroundState(grid, atom)
+    t_host = time.perf_counter() - t0
+    stats = {"workload": "%s LDA ground state, N=%d r_max=%g, tol 1e-11" % (ATOM, N, R_MAX),
+             "device_s": t_dev, "device_seconds": dev.seconds, "device_iterations": dev.iterations,
+             "host_s": t_host, "host_iterations": host.iterations, "host_cores": os.cpu_count(),
+             "max_abs_diff_eps": float(np.abs(dev.eps - host.eps).max()),
+             "max_abs_diff_v_gs": float(np.abs(dev.v_gs - host.v_gs).max())}
+    log("ground state: device %.3f s (%d it), host %.2f s (%d it)"
+        % (t_dev, dev.iterations, t_host, host.iterations))
+    return stats, dev
+
+
 def measured_traffic():
     """HBM bytes per launch of the roofline kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
@@ -242,10 +267,11 @@ def main():
     from ggbtddft_amd.pulse import Pulse
     from ggbtddft_amd.system import AtomSystem
 
+    gs_stats, gs_dev = ground_state_timing()
     t0 = time.perf_counter()
-    sysm = AtomSystem(ATOM, N, R_MAX, LMAX, DT)
-    log("precompute %.1f s (ground state %d iterations, sum nkeep %d)"
-        % (time.perf_counter() - t0, sysm.gs.iterations, int(sysm.nkeep.sum())))
+    sysm = AtomSystem(ATOM, N, R_MAX, LMAX, DT, ground_state=gs_dev)
+    log("precompute %.1f s after the ground state (sum nkeep %d)"
+        % (time.perf_counter() - t0, int(sysm.nkeep.sum())))
     pulse = Pulse(INTENSITY, WAVELENGTH, CYCLES)
     nfull = pulse.n_steps(DT)
     need = args.warmup + args.steps
@@ -300,7 +326,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded imaginary-time ground state, analytic sin^2 pulse)",
+        "data": "synthetic (device LDA ground state from hydrogenic start, analytic sin^2 pulse)",
         "config": {"workload": "argon all-electron FP64 plain, one instance",
                    "atom": ATOM, "n_r": N, "r_max": R_MAX, "lmax": LMAX, "l_hartree": 2 * LMAX,
                    "n_theta": sysm.n_theta, "orbitals": norb, "dt": DT,
@@ -321,6 +347,7 @@ def main():
     if args.serving > 0:
         line["serving"] = serving_run(sysm, args)
     line["spectrum"] = spectrum_timing(nfull)
+    line["ground_state"] = gs_stats
     if args.precision_report:
         line["precision_report"] = precision_report(sysm, pulse)
     if not args.no_cpu_baseline:

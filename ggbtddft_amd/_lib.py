@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libggbtd.so")
 
-GGB_ABI_VERSION = 2
+GGB_ABI_VERSION = 3
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "call out of order",
           -4: "device allocation failed"}
 
@@ -20,8 +20,8 @@ EXPORTS = ("ggb_abi_version", "ggb_create", "ggb_upload_static", "ggb_upload_orb
            "ggb_debug_substep", "ggb_set_timing", "ggb_get_timing", "ggb_selftest_mfma",
            "ggb_field_ring", "ggb_set_field_slice", "ggb_upload_columns",
            "ggb_rebuild_potential", "ggb_footprint", "ggb_selftest_copy", "ggb_set_graph",
-           "ggb_spectrum", "ggb_set_kinetic_variant", "ggb_autotune", "ggb_last_error",
-           "ggb_destroy")
+           "ggb_spectrum", "ggb_set_kinetic_variant", "ggb_autotune", "ggb_ground_state",
+           "ggb_last_error", "ggb_destroy")
 KIN_VARIANTS = ("default", "narrow", "tiled16", "tiled32")
 NPHASE = 5
 PHASES = ("half_potential_laser", "kinetic_qt", "kinetic_q", "laser_half_potential_obs",
@@ -47,6 +47,13 @@ class Static(ctypes.Structure):
                 ("lam", _dp), ("nkeep", _ip), ("B", _dp), ("PL", _dp), ("chi", _dp),
                 ("omega", _dp), ("m", _ip), ("occ", _dp), ("rho_frozen", _dp),
                 ("phi", _dp), ("l0", _ip), ("eps", _dp)]
+
+
+class GsProblem(ctypes.Structure):
+    _fields_ = [("n_r", ctypes.c_int32), ("n_shell", ctypes.c_int32), ("sweeps", ctypes.c_int32),
+                ("max_iter", ctypes.c_int32), ("interacting", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("Z", ctypes.c_double), ("mix", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("tol", ctypes.c_double)]
 
 
 class GGBError(RuntimeError):
@@ -108,6 +115,9 @@ def load():
     lib.ggb_spectrum.argtypes = [ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_double,
                                  ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
                                  _dp, ctypes.c_int32, _dp, _dp]
+    lib.ggb_ground_state.restype = ctypes.c_int
+    lib.ggb_ground_state.argtypes = [ctypes.POINTER(GsProblem), _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     _dp, _dp, _dp, _dp, _ip, _ip]
     lib.ggb_set_graph.restype = ctypes.c_int
     lib.ggb_set_graph.argtypes = [vp, ctypes.c_int32]
     lib.ggb_selftest_mfma.restype = ctypes.c_int

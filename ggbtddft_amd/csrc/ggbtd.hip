@@ -2434,3 +2434,404 @@ extern "C" int ggb_selftest_copy(int32_t device, uint64_t bytes, double *ms) {
     (void)hipFree(b);
     return e == hipSuccess ? GGB_OK : GGB_E_HIP;
 }
+
+// ---------------------------------------------------------------------------
+// Ground state on the device: the step before the propagation (a15's
+// orbitals and v_gs).  The host iteration it replaces is the imaginary-time
+// iteration with the exact exponential of the frozen-potential radial
+// Hamiltonian (one dense eigendecomposition per l and iteration); the
+// reference computes its ground state by diagonalisation in
+// src/scripts/phys/groundState.py:182-457.  Both converge to the same fixed
+// point, the self-consistent spherical LDA ground state of the discretised
+// problem: one real radial function u_s (u units) per closed (n, l) shell.
+// One SCF iteration:
+//   k_gs_potential  rho_u = sum_s occ_s u_s^2,  v_out = V_0[rho_u] + v_xc[rho],
+//                   v <- v + mix (v_out - v)   (v = v_out at the first)
+//   k_gs_sweep      one work-group per l group, `sweeps` passes over its
+//                   shells in shell order; for shell s:
+//                     x <- u_s made orthogonal to the group's lower shells
+//                          (modified Gram-Schmidt) and normalised
+//                     r  = H x - theta x,  theta = x.Hx,
+//                          H = -T/2 + l(l+1)/2r^2 - Z/r + v
+//                     w  = Q0 diag(1/(|lam0 - theta| + gamma)) Q0^T r, with
+//                          (Q0, lam0) the eigenpairs of the bare-nucleus H
+//                          (v = 0) of this l, orthogonalised to the lower
+//                          shells and to x, normalised
+//                     u_s <- the lower Ritz vector of span(x, w)
+//                   eps_s = u_s.H u_s
+// until max_s |eps_s - eps_s(prev)| < tol and max |v_out - v| < 1e3 tol (the
+// host iteration's test).  An exact eigenvector has r = 0 and is left as it
+// is, so the fixed point is the one of the exact iteration; the spectral
+// preconditioner (positive definite) only sets the rate.  All FP64.
+// ---------------------------------------------------------------------------
+#define GS_WG 512
+#define GS_MAX_N 1536   // k_gs_sweep keeps 5 N-vectors in LDS (60 KiB)
+#define GS_MAX_K 8      // shells per l group
+
+struct GsArgs {
+    const double *r, *wr, *T, *Q0, *Q0T, *lam0, *occ;
+    const int32_t *gsh;  // [n_group][GS_MAX_K] shells of each l group, shell order, -1 padded
+    const int32_t *gl;   // [n_group] l of each group
+    double *u;           // [n_shell][N]
+    double *eps;         // [n_shell] (first iteration: +inf)
+    double *v, *rho_u;   // [N]
+    double *res;         // [0] max |v_out - v|, [1 + g] max |eps - eps_prev| of group g
+    int N, n_shell, n_group, sweeps, interacting, first, final_;
+    double Z, mix, gamma;
+};
+
+// sum over the work-group, fixed order (every thread gets the result)
+static __device__ double gs_sum(double v, double *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();  // earlier readers of red are done
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+    return s;
+}
+
+static __device__ double gs_max(double v, double *red) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off));
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s = fmax(s, red[k]);
+    return s;
+}
+
+// inclusive scan (dir 0: prefix, 1: suffix) of one value per thread, Hillis-Steele in LDS
+static __device__ double gs_scan(double v, int dir, double *buf) {
+    const int t = threadIdx.x, n = blockDim.x;
+    const int me = dir ? n - 1 - t : t;
+    __syncthreads();
+    buf[me] = v;
+    __syncthreads();
+    for (int off = 1; off < n; off <<= 1) {
+        const double o = me >= off ? buf[me - off] : 0.0;
+        __syncthreads();
+        buf[me] += o;
+        __syncthreads();
+    }
+    return buf[me];
+}
+
+// rho_u, V_0 + v_xc and the damped update (final_: v <- v_out, no mixing)
+__global__ __launch_bounds__(GS_WG) void k_gs_potential(GsArgs a) {
+    __shared__ double buf[GS_WG];
+    __shared__ double red[GS_WG / 64];
+    const int t = threadIdx.x, nt = blockDim.x, N = a.N;
+    // thread t owns the radial chunk [i0, i1)
+    const int ch = (N + nt - 1) / nt;
+    const int i0 = min(N, t * ch), i1 = min(N, i0 + ch);
+    double sq = 0.0, sqr = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        double ru = 0.0;
+        for (int s = 0; s < a.n_shell; ++s) {
+            const double x = a.u[(size_t)s * N + i];
+            ru += a.occ[s] * x * x;
+        }
+        a.rho_u[i] = ru;
+        sq += ru;
+        sqr += ru / a.r[i];
+    }
+    // V_0(r_i) = (1/r_i) sum_{b <= i} q_b + sum_{b > i} q_b / r_b
+    const double pre = gs_scan(sq, 0, buf) - sq;    // sum over the earlier chunks
+    const double suf = gs_scan(sqr, 1, buf) - sqr;  // sum over the later chunks
+    double run = pre, dmax = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        const double ru = a.rho_u[i], ri = a.r[i];
+        run += ru;
+        double tail = suf;
+        for (int b = i + 1; b < i1; ++b) tail += a.rho_u[b] / a.r[b];
+        double vo = 0.0;
+        if (a.interacting) {
+            const double rho = ru / (4.0 * M_PI * a.wr[i] * ri * ri);
+            vo = run / ri + tail + lda_vxc<double>(rho);
+        }
+        const double vn = (a.final_ || a.first) ? vo : a.v[i] + a.mix * (vo - a.v[i]);
+        a.v[i] = vn;
+        dmax = fmax(dmax, fabs(vo - vn));
+    }
+    dmax = gs_max(dmax, red);
+    if (t == 0) a.res[0] = dmax;
+}
+
+// y = H x for the group's l (T symmetric: thread i reads column i, coalesced)
+static __device__ void gs_apply_h(const GsArgs &a, int l, const double *x, double *y) {
+    const int N = a.N;
+    const double cl = 0.5 * (double)l * (double)(l + 1);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double *Tc = a.T + i;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int j = 0;
+        for (; j + 4 <= N; j += 4) {
+            s0 += Tc[(size_t)j * N] * x[j];
+            s1 += Tc[(size_t)(j + 1) * N] * x[j + 1];
+            s2 += Tc[(size_t)(j + 2) * N] * x[j + 2];
+            s3 += Tc[(size_t)(j + 3) * N] * x[j + 3];
+        }
+        for (; j < N; ++j) s0 += Tc[(size_t)j * N] * x[j];
+        const double ri = a.r[i];
+        const double d = cl / (ri * ri) - a.Z / ri + a.v[i];
+        y[i] = -0.5 * ((s0 + s1) + (s2 + s3)) + d * x[i];
+    }
+}
+
+// w = Q0 diag(1/(|lam0 - theta| + gamma)) Q0^T r, through tmp (all in LDS)
+static __device__ void gs_precondition(const GsArgs &a, int g, double theta, const double *rv,
+                                       double *tmp, double *w) {
+    const int N = a.N;
+    const double *Q = a.Q0 + (size_t)g * N * N;    // [i][k]
+    const double *QT = a.Q0T + (size_t)g * N * N;  // [k][i]
+    const double *lam = a.lam0 + (size_t)g * N;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        double s0 = 0.0, s1 = 0.0;
+        int i = 0;
+        for (; i + 2 <= N; i += 2) {
+            s0 += Q[(size_t)i * N + k] * rv[i];
+            s1 += Q[(size_t)(i + 1) * N + k] * rv[i + 1];
+        }
+        for (; i < N; ++i) s0 += Q[(size_t)i * N + k] * rv[i];
+        tmp[k] = (s0 + s1) / (fabs(lam[k] - theta) + a.gamma);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double s0 = 0.0, s1 = 0.0;
+        int k = 0;
+        for (; k + 2 <= N; k += 2) {
+            s0 += QT[(size_t)k * N + i] * tmp[k];
+            s1 += QT[(size_t)(k + 1) * N + i] * tmp[k + 1];
+        }
+        for (; k < N; ++k) s0 += QT[(size_t)k * N + i] * tmp[k];
+        w[i] = s0 + s1;
+    }
+    __syncthreads();
+}
+
+static __device__ double gs_dot(const double *x, const double *y, int N, double *red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s += x[i] * y[i];
+    return gs_sum(s, red);
+}
+
+// x -= (q.x) q for each lower shell q of the group, in shell order
+static __device__ void gs_deflate(const GsArgs &a, const int32_t *sh, int j, double *x, double *red) {
+    const int N = a.N;
+    for (int q = 0; q < j; ++q) {
+        const double *uq = a.u + (size_t)sh[q] * N;
+        const double c = gs_dot(uq, x, N, red);
+        for (int i = threadIdx.x; i < N; i += blockDim.x) x[i] -= c * uq[i];
+        __syncthreads();
+    }
+}
+
+// one work-group per l group: `sweeps` preconditioned Rayleigh-Ritz passes
+// over the group's shells with the potential a.v; eps and the group's
+// largest eps change into a.res[1 + g]
+__global__ __launch_bounds__(GS_WG) void k_gs_sweep(GsArgs a) {
+    extern __shared__ double gsm[];
+    __shared__ double red[GS_WG / 64];
+    const int g = blockIdx.x, N = a.N, l = a.gl[g];
+    const int32_t *sh = a.gsh + (size_t)g * GS_MAX_K;
+    int k = 0;
+    while (k < GS_MAX_K && sh[k] >= 0) ++k;
+    double *x = gsm, *hx = gsm + N, *w = gsm + 2 * N, *hw = gsm + 3 * N, *tmp = gsm + 4 * N;
+    for (int sw = 0; sw < a.sweeps; ++sw) {
+        for (int j = 0; j < k; ++j) {
+            double *us = a.u + (size_t)sh[j] * N;
+            for (int i = threadIdx.x; i < N; i += blockDim.x) x[i] = us[i];
+            __syncthreads();
+            gs_deflate(a, sh, j, x, red);
+            const double nx = sqrt(gs_dot(x, x, N, red));
+            for (int i = threadIdx.x; i < N; i += blockDim.x) x[i] /= nx;
+            __syncthreads();
+            gs_apply_h(a, l, x, hx);
+            __syncthreads();
+            const double theta = gs_dot(x, hx, N, red);
+            for (int i = threadIdx.x; i < N; i += blockDim.x) hw[i] = hx[i] - theta * x[i];  // residual
+            __syncthreads();
+            gs_precondition(a, g, theta, hw, tmp, w);
+            gs_deflate(a, sh, j, w, red);
+            const double cx = gs_dot(x, w, N, red);
+            for (int i = threadIdx.x; i < N; i += blockDim.x) w[i] -= cx * x[i];
+            __syncthreads();
+            const double nw = sqrt(gs_dot(w, w, N, red));
+            // new x = ax x + aw w (and H x alike)
+            double ax = 1.0, aw = 0.0;
+            if (nw > 1e-300) {
+                for (int i = threadIdx.x; i < N; i += blockDim.x) w[i] /= nw;
+                __syncthreads();
+                gs_apply_h(a, l, w, hw);
+                __syncthreads();
+                const double b = gs_dot(x, hw, N, red);
+                const double c = gs_dot(w, hw, N, red);
+                // Jacobi rotation of [[theta, b], [b, c]] (t = tan of the angle):
+                // eigenvalues theta - t b with vector (cs, -sn) and c + t b with
+                // vector (sn, cs); the lower one is kept
+                double tt = 0.0;
+                if (b != 0.0) {
+                    const double zeta = (c - theta) / (2.0 * b);
+                    tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                }
+                const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = tt * cs;
+                if (c + tt * b < theta - tt * b)
+                    ax = sn, aw = cs;
+                else
+                    ax = cs, aw = -sn;
+            }
+            for (int i = threadIdx.x; i < N; i += blockDim.x) {
+                const double xn = ax * x[i] + aw * w[i];
+                x[i] = xn;
+                hx[i] = ax * hx[i] + aw * hw[i];
+                us[i] = xn;
+            }
+            __syncthreads();
+        }
+    }
+    // orbital energies with the final vectors, and the largest change
+    double dmax = 0.0;
+    for (int j = 0; j < k; ++j) {
+        const double *us = a.u + (size_t)sh[j] * N;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) x[i] = us[i];
+        __syncthreads();
+        gs_apply_h(a, l, x, hx);
+        __syncthreads();
+        const double e = gs_dot(x, hx, N, red);
+        const double prev = a.eps[sh[j]];
+        dmax = fmax(dmax, fabs(e - prev));  // +inf at the first iteration
+        __syncthreads();
+        if (threadIdx.x == 0) a.eps[sh[j]] = e;
+    }
+    if (threadIdx.x == 0) a.res[1 + g] = dmax;
+}
+
+extern "C" int ggb_ground_state(const ggb_gs_problem *p, const int32_t *shell_l, const double *occ,
+                                const double *r, const double *wr, const double *T,
+                                const double *Q0, const double *lam0, double *u, double *eps,
+                                double *v_gs, double *rho_u, int32_t *iterations,
+                                int32_t *converged) {
+    auto bad = [](const char *msg) {
+        g_create_err = std::string("ggb_ground_state: ") + msg;
+        return GGB_E_ARG;
+    };
+    if (!p || !shell_l || !occ || !r || !wr || !T || !Q0 || !lam0 || !u || !eps || !v_gs || !rho_u ||
+        !iterations || !converged)
+        return bad("null argument");
+    const int N = p->n_r, ns = p->n_shell;
+    if (N < 4 || N > GS_MAX_N) return bad("n_r outside 4..1536");
+    if (ns < 1 || ns > 64) return bad("n_shell outside 1..64");
+    if (p->sweeps < 1 || p->max_iter < 1 || !(p->tol > 0.0) || !(p->mix > 0.0 && p->mix <= 1.0) ||
+        !(p->gamma > 0.0) || !(p->Z > 0.0))
+        return bad("sweeps, max_iter, tol, mix (0, 1], gamma or Z out of range");
+    // l groups: the distinct l of the shells, ascending; shells of a group in shell order
+    std::vector<int> ls;
+    for (int s = 0; s < ns; ++s) {
+        if (shell_l[s] < 0 || shell_l[s] > 127 || !(occ[s] >= 0.0)) return bad("shell l or occupation");
+        if (std::find(ls.begin(), ls.end(), shell_l[s]) == ls.end()) ls.push_back(shell_l[s]);
+    }
+    std::sort(ls.begin(), ls.end());
+    const int G = (int)ls.size();
+    std::vector<int32_t> gsh((size_t)G * GS_MAX_K, -1), gl(G);
+    for (int g = 0; g < G; ++g) {
+        gl[g] = ls[g];
+        int k = 0;
+        for (int s = 0; s < ns; ++s)
+            if (shell_l[s] == ls[g]) {
+                if (k >= GS_MAX_K) return bad("more than 8 shells of one l");
+                if (k >= N) return bad("more shells of one l than radial nodes");
+                gsh[(size_t)g * GS_MAX_K + k++] = s;
+            }
+    }
+    if (hipSetDevice(p->device) != hipSuccess) {
+        g_create_err = "ggb_ground_state: hipSetDevice failed";
+        return GGB_E_HIP;
+    }
+    const size_t NN = (size_t)N * N;
+    // transposed eigenvector tables and the start: u_s = the k-th bare-nucleus
+    // eigenvector of its l (k = rank of s within its group)
+    std::vector<double> q0t((size_t)G * NN), u0((size_t)ns * N), einit(ns, INFINITY);
+    for (int g = 0; g < G; ++g) {
+        const double *Qg = Q0 + (size_t)g * NN;
+        double *QTg = q0t.data() + (size_t)g * NN;
+        for (int i = 0; i < N; ++i)
+            for (int a = 0; a < N; ++a) QTg[(size_t)a * N + i] = Qg[(size_t)i * N + a];
+        for (int k = 0; k < GS_MAX_K && gsh[(size_t)g * GS_MAX_K + k] >= 0; ++k) {
+            const int s = gsh[(size_t)g * GS_MAX_K + k];
+            for (int i = 0; i < N; ++i) u0[(size_t)s * N + i] = Qg[(size_t)i * N + k];
+        }
+    }
+    hipStream_t st = nullptr;
+    double *dr = nullptr, *dwr = nullptr, *dT = nullptr, *dQ = nullptr, *dQT = nullptr, *dlam = nullptr,
+           *docc = nullptr, *du = nullptr, *deps = nullptr, *dv = nullptr, *drho = nullptr, *dres = nullptr;
+    int32_t *dgsh = nullptr, *dgl = nullptr;
+    int rc = GGB_OK;
+    auto hip = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == GGB_OK) {
+            g_create_err = std::string("ggb_ground_state: ") + what + ": " + hipGetErrorString(e);
+            rc = e == hipErrorOutOfMemory ? GGB_E_NOMEM : GGB_E_HIP;
+        }
+        return rc == GGB_OK;
+    };
+    auto up = [&](void *d, const void *h, size_t bytes) {
+        return hip(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st), "upload");
+    };
+    int it = 0;
+    bool conv = false;
+    if (hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream") && hip(dalloc(&dr, N), "alloc") &&
+        hip(dalloc(&dwr, N), "alloc") && hip(dalloc(&dT, NN), "alloc") &&
+        hip(dalloc(&dQ, (size_t)G * NN), "alloc") && hip(dalloc(&dQT, (size_t)G * NN), "alloc") &&
+        hip(dalloc(&dlam, (size_t)G * N), "alloc") && hip(dalloc(&docc, ns), "alloc") &&
+        hip(dalloc(&du, (size_t)ns * N), "alloc") && hip(dalloc(&deps, ns), "alloc") &&
+        hip(dalloc(&dv, N), "alloc") && hip(dalloc(&drho, N), "alloc") && hip(dalloc(&dres, 1 + G), "alloc") &&
+        hip(dalloc(&dgsh, (size_t)G * GS_MAX_K), "alloc") && hip(dalloc(&dgl, G), "alloc") &&
+        up(dr, r, N * 8) && up(dwr, wr, N * 8) && up(dT, T, NN * 8) && up(dQ, Q0, (size_t)G * NN * 8) &&
+        up(dQT, q0t.data(), (size_t)G * NN * 8) && up(dlam, lam0, (size_t)G * N * 8) &&
+        up(docc, occ, ns * 8) && up(du, u0.data(), (size_t)ns * N * 8) && up(deps, einit.data(), ns * 8) &&
+        up(dgsh, gsh.data(), gsh.size() * 4) && up(dgl, gl.data(), G * 4)) {
+        GsArgs a;
+        a.r = dr, a.wr = dwr, a.T = dT, a.Q0 = dQ, a.Q0T = dQT, a.lam0 = dlam, a.occ = docc;
+        a.gsh = dgsh, a.gl = dgl, a.u = du, a.eps = deps, a.v = dv, a.rho_u = drho, a.res = dres;
+        a.N = N, a.n_shell = ns, a.n_group = G, a.sweeps = p->sweeps, a.interacting = p->interacting ? 1 : 0;
+        a.Z = p->Z, a.mix = p->mix, a.gamma = p->gamma;
+        a.final_ = 0;
+        const size_t lds = (size_t)5 * N * sizeof(double);
+        std::vector<double> res(1 + G);
+        for (it = 1; it <= p->max_iter; ++it) {
+            a.first = it == 1;
+            k_gs_potential<<<1, GS_WG, 0, st>>>(a);
+            k_gs_sweep<<<G, GS_WG, lds, st>>>(a);
+            if (!hip(hipGetLastError(), "launch") ||
+                !hip(hipMemcpyAsync(res.data(), dres, (1 + G) * 8, hipMemcpyDeviceToHost, st), "download") ||
+                !hip(hipStreamSynchronize(st), "sync"))
+                break;
+            double de = 0.0;
+            for (int g = 0; g < G; ++g) de = std::max(de, res[1 + g]);
+            if (de < p->tol && res[0] < 1e3 * p->tol) {
+                conv = true;
+                break;
+            }
+        }
+        if (it > p->max_iter) it = p->max_iter;
+        if (rc == GGB_OK) {
+            a.first = 0, a.final_ = 1;  // v_gs = v_out of the final orbitals
+            k_gs_potential<<<1, GS_WG, 0, st>>>(a);
+            (void)(hip(hipGetLastError(), "launch") &&
+                   hip(hipMemcpyAsync(u, du, (size_t)ns * N * 8, hipMemcpyDeviceToHost, st), "download") &&
+                   hip(hipMemcpyAsync(eps, deps, ns * 8, hipMemcpyDeviceToHost, st), "download") &&
+                   hip(hipMemcpyAsync(v_gs, dv, N * 8, hipMemcpyDeviceToHost, st), "download") &&
+                   hip(hipMemcpyAsync(rho_u, drho, N * 8, hipMemcpyDeviceToHost, st), "download") &&
+                   hip(hipStreamSynchronize(st), "sync"));
+        }
+    }
+    for (void *d : {(void *)dr, (void *)dwr, (void *)dT, (void *)dQ, (void *)dQT, (void *)dlam, (void *)docc,
+                    (void *)du, (void *)deps, (void *)dv, (void *)drho, (void *)dres, (void *)dgsh, (void *)dgl})
+        if (d) (void)hipFree(d);
+    if (st) (void)hipStreamDestroy(st);
+    *iterations = it;
+    *converged = conv ? 1 : 0;
+    return rc;
+}

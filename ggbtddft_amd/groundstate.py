@@ -13,6 +13,9 @@ shell, shells in atom order).  Each iteration:
     Gram-Schmidt within each l in shell order
 until the shell energies change by less than tol and v_out = v to 1e3 tol.
 """
+import ctypes
+import time
+
 import numpy as np
 
 from . import lda
@@ -88,3 +91,59 @@ class GroundState:
         self.eps = eps
         self.rho_u = occ @ (np.abs(u) ** 2)
         self.v_gs = spherical_potential(grid, self.rho_u, self.interacting)
+
+
+class DeviceGroundState:
+    """The same ground state computed on the GPU (ggb_ground_state, FP64).
+
+    The device reaches the fixed point of the iteration above by
+    preconditioned Rayleigh-Ritz sweeps instead of one dense eigendecomposition
+    per l and iteration: per SCF iteration the damped potential update, then
+    ``sweeps`` passes over the shells of each l (lower Ritz vector of
+    span(u, w), w the residual H u - eps u preconditioned by the bare-nucleus
+    spectral form Q0 diag(1/(|lam0 - eps| + gamma)) Q0^T).  The host computes
+    only the bare-nucleus eigenpairs of each l present (the preconditioner and
+    the start, independent of v).  Same attributes as GroundState; u is real.
+    """
+
+    def __init__(self, grid, atom, interacting=None, dtau=None, mix=0.3, tol=1e-11,
+                 max_iter=3000, sweeps=8, gamma=2.0, device=0):
+        from . import _lib
+        self.grid, self.atom = grid, atom
+        self.interacting = (atom.symbol != "H") if interacting is None else bool(interacting)
+        N = grid.N
+        shells = atom.shells
+        shell_l = np.array([l for (_, l, _) in shells], dtype=np.int32)
+        occ = np.array([o for (_, _, o) in shells], dtype=np.float64)
+        ls = sorted(set(int(l) for l in shell_l))
+        t0 = time.perf_counter()
+        Q0 = np.empty((len(ls), N, N))
+        lam0 = np.empty((len(ls), N))
+        for g, l in enumerate(ls):
+            lam0[g], Q0[g] = np.linalg.eigh(grid.hamiltonian(l, atom.Z))
+        p = _lib.GsProblem(n_r=N, n_shell=len(shells), sweeps=int(sweeps), max_iter=int(max_iter),
+                           interacting=int(self.interacting), device=int(device), Z=float(atom.Z),
+                           mix=float(mix), gamma=float(gamma), tol=float(tol))
+        u = np.zeros((len(shells), N))
+        eps = np.zeros(len(shells))
+        v_gs = np.zeros(N)
+        rho_u = np.zeros(N)
+        it = np.zeros(1, dtype=np.int32)
+        conv = np.zeros(1, dtype=np.int32)
+        lib = _lib.load()
+        t1 = time.perf_counter()
+        rc = lib.ggb_ground_state(ctypes.byref(p), _lib.iptr(shell_l), _lib.dptr(occ),
+                                  _lib.dptr(np.ascontiguousarray(grid.r)),
+                                  _lib.dptr(np.ascontiguousarray(grid.wr)),
+                                  _lib.dptr(np.ascontiguousarray(grid.T)), _lib.dptr(Q0),
+                                  _lib.dptr(lam0), _lib.dptr(u), _lib.dptr(eps), _lib.dptr(v_gs),
+                                  _lib.dptr(rho_u), _lib.iptr(it), _lib.iptr(conv))
+        _lib.check(rc, None, "ggb_ground_state")
+        t2 = time.perf_counter()
+        self.seconds = {"host_bare_eigenpairs": t1 - t0, "device_iteration": t2 - t1}
+        self.u = u.astype(np.complex128)
+        self.eps = eps
+        self.v_gs = v_gs
+        self.rho_u = rho_u
+        self.iterations = int(it[0])
+        self.converged = bool(conv[0])

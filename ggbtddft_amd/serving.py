@@ -151,7 +151,11 @@ class Server:
         self.factory = factory or functools.partial(engine.Propagator, precision=self.precision)
         self.footprint = (lambda sysm: engine.footprint(sysm, slots, ring, chunk, self.precision)) \
             if factory is None else (lambda sysm: 0)
-        self.system_factory = system_factory or (lambda k: AtomSystem(k[0], k[1], k[2], k[3], k[4]))
+        # with the GPU propagator the ground state of a new configuration is
+        # computed on the device too
+        gs = "device" if factory is None else None
+        self.system_factory = system_factory or (
+            lambda k: AtomSystem(k[0], k[1], k[2], k[3], k[4], ground_state=gs, gs_device=device))
         self.systems = {}
         self.lanes = {}
         self.used = 0

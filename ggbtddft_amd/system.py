@@ -21,7 +21,7 @@ import numpy as np
 from .angular import AngularTables
 from .atoms import Atom
 from .grid import RadialGrid
-from .groundstate import GroundState
+from .groundstate import DeviceGroundState, GroundState
 
 
 def manolopoulos_cap(r, r_mask, r_max):
@@ -44,7 +44,7 @@ def _round16(n):
 
 class AtomSystem:
     def __init__(self, atom, N, r_max, lmax, dt, zeta=1.0, n_theta=None, l_h=None, seed=0,
-                 cap=True, interacting=None, ecut_factor=1.0, ground_state=None):
+                 cap=True, interacting=None, ecut_factor=1.0, ground_state=None, gs_device=0):
         self.atom = atom if isinstance(atom, Atom) else Atom(atom)
         self.N, self.r_max, self.lmax, self.dt = int(N), float(r_max), int(lmax), float(dt)
         self.n_l = self.lmax + 1
@@ -57,8 +57,15 @@ class AtomSystem:
         self.m = np.array([o[2] for o in orbs], dtype=np.int32)
         self.occ = np.array([o[3] for o in orbs], dtype=np.float64)
         self.angular = AngularTables(self.lmax, self.n_theta, self.l_h, self.m)
-        self.gs = ground_state if ground_state is not None else GroundState(
-            self.grid, self.atom, interacting=interacting, seed=seed)
+        # ground_state: None (host iteration), "device" (ggb_ground_state on
+        # gs_device) or a computed ground state of this grid and atom
+        if ground_state is None:
+            self.gs = GroundState(self.grid, self.atom, interacting=interacting, seed=seed)
+        elif isinstance(ground_state, str) and ground_state == "device":
+            self.gs = DeviceGroundState(self.grid, self.atom, interacting=interacting,
+                                        device=gs_device)
+        else:
+            self.gs = ground_state
         self.interacting = self.gs.interacting
         self.v_gs = self.gs.v_gs
         self.ecut = ecut_factor * math.pi / self.dt

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define GGB_ABI_VERSION 2
+#define GGB_ABI_VERSION 3
 
 #define GGB_OK 0
 #define GGB_E_ARG (-1)      /* invalid argument / size */
@@ -224,6 +224,37 @@ int ggb_selftest_copy(int32_t device, uint64_t bytes, double *ms);
 int ggb_spectrum(int32_t device, const double *signal, int32_t n, double dt,
                  int32_t differentiate, double beta, int32_t n_pad, double omega0,
                  const double *orders, int32_t n_orders, double *power, double *intensity);
+
+/* Ground state on the device: the orbitals and v_gs that the propagation
+ * starts from.  The reference loads them from workingData/
+ * (h_initialiseTDKSOSet.cuh:19-107), written by its diagonalisation SCF
+ * (src/scripts/phys/groundState.py:182-457); the host counterpart here is the
+ * imaginary-time iteration of ggbtddft_amd/groundstate.py, whose fixed point
+ * (the self-consistent spherical LDA ground state on the grid) this call
+ * reaches by preconditioned Rayleigh-Ritz sweeps on the device, FP64.
+ * Inputs: shell_l/occ [n_shell] (atom order), the grid's r, wr (= Lobatto
+ * weight * dr/dx) [n_r] and kinetic matrix T [n_r][n_r] (symmetric), and for
+ * each distinct l of the shells, ascending, the eigenvectors
+ * Q0[g][i][k] (column k = k-th eigenvector) and eigenvalues lam0[g][k] of
+ * the bare-nucleus radial Hamiltonian -T/2 + l(l+1)/2r^2 - Z/r (the
+ * preconditioner, and the start: shell k of each l starts as column k).
+ * Outputs: u[n_shell][n_r] (real, unit norm, u units), eps[n_shell], v_gs and
+ * rho_u [n_r] of the final orbitals, the iteration count and whether the
+ * host iteration's test (|d eps| < tol, |v_out - v| < 1e3 tol) was met. */
+typedef struct {
+    int32_t n_r;         /* 4..1536 */
+    int32_t n_shell;     /* 1..64, at most 8 of one l */
+    int32_t sweeps;      /* preconditioned passes over each l's shells per iteration (8) */
+    int32_t max_iter;    /* SCF iterations at most */
+    int32_t interacting; /* 0: bare nucleus, v = 0 */
+    int32_t device;
+    double Z, mix, gamma, tol; /* nuclear charge, potential damping (0.3), preconditioner shift (2), 1e-11 */
+} ggb_gs_problem;
+
+int ggb_ground_state(const ggb_gs_problem *p, const int32_t *shell_l, const double *occ,
+                     const double *r, const double *wr, const double *T, const double *Q0,
+                     const double *lam0, double *u, double *eps, double *v_gs, double *rho_u,
+                     int32_t *iterations, int32_t *converged);
 
 const char *ggb_last_error(const ggb_ctx *ctx);
 

@@ -126,3 +126,35 @@ def test_spectrum_rejects_bad_arguments_without_device(lib):
     rc = lib.ggb_spectrum(0, _lib.dptr(x), 8, 0.05, 0, 24.0, 8, 0.0, None, 0, None, None)
     assert rc == -1  # GGB_E_ARG
     assert b"ggb_spectrum" in lib.ggb_last_error(None)
+
+
+def test_ground_state_rejects_bad_arguments_without_device(lib):
+    # argument checks come before any device call
+    N = 8
+    z = np.zeros(N)
+    zz = np.zeros((N, N))
+    sl = np.zeros(1, dtype=np.int32)
+    it = np.zeros(1, dtype=np.int32)
+    conv = np.zeros(1, dtype=np.int32)
+    occ = np.ones(1)
+    out = np.zeros(N)
+    base = dict(n_r=N, n_shell=1, sweeps=8, max_iter=10, interacting=1, device=0, Z=2.0,
+                mix=0.3, gamma=2.0, tol=1e-11)
+    for bad in (dict(n_r=2), dict(n_r=4096), dict(n_shell=0), dict(sweeps=0), dict(mix=1.5),
+                dict(gamma=0.0), dict(tol=0.0), dict(Z=-1.0)):
+        p = _lib.GsProblem(**{**base, **bad})
+        rc = lib.ggb_ground_state(ctypes.byref(p), _lib.iptr(sl), _lib.dptr(occ), _lib.dptr(z),
+                                  _lib.dptr(z), _lib.dptr(zz), _lib.dptr(zz), _lib.dptr(z),
+                                  _lib.dptr(out), _lib.dptr(occ.copy()), _lib.dptr(out.copy()),
+                                  _lib.dptr(out.copy()), _lib.iptr(it), _lib.iptr(conv))
+        assert rc == -1, bad
+        assert b"ggb_ground_state" in lib.ggb_last_error(None)
+    # nine shells of one l exceed the per-l limit
+    p = _lib.GsProblem(**{**base, "n_shell": 9})
+    sl9 = np.zeros(9, dtype=np.int32)
+    occ9 = np.ones(9)
+    rc = lib.ggb_ground_state(ctypes.byref(p), _lib.iptr(sl9), _lib.dptr(occ9), _lib.dptr(z),
+                              _lib.dptr(z), _lib.dptr(zz), _lib.dptr(zz), _lib.dptr(z),
+                              _lib.dptr(np.zeros(9 * N)), _lib.dptr(np.zeros(9)), _lib.dptr(out.copy()),
+                              _lib.dptr(out.copy()), _lib.iptr(it), _lib.iptr(conv))
+    assert rc == -1
