@@ -31,6 +31,7 @@
 //              row = 4 (L / 16) + r for v_mfma_f32_16x16x4f32
 //   (mfma_t<T>::row).
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1866,8 +1867,22 @@ static int kin_variant(const ggb_ctx *ctx) {
 
 // fresh: the angular first half has just written psi32 (all blocks but the
 // ground-state ones); otherwise pass 1 converts every block itself
+// launch of a kinetic product kernel; with events (timing) through
+// hipExtLaunchKernel, which stamps e0/e1 with the kernel's own start and end
+template <typename... KArgs, typename... Args>
+static void kin_launch(void (*k)(KArgs...), dim3 g, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                       Args... args) {
+    if (e0)
+        hipExtLaunchKernelGGL(k, g, dim3(WG), 0, st, e0, e1, 0, args...);
+    else
+        k<<<g, WG, 0, st>>>(args...);
+}
+
+// e0/e1 (optional): events stamped with the start and end of the pass's
+// product kernel itself, for the kernel-duration timing
 template <typename T>
-static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh) {
+static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh, hipEvent_t e0 = nullptr,
+                             hipEvent_t e1 = nullptr) {
     const ggb_problem &p = ctx->p;
     const T *Q = reinterpret_cast<const T *>(ctx->Q);
     T *Y = reinterpret_cast<T *>(ctx->Y);
@@ -1887,37 +1902,47 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh) {
                 (const double2 *)ctx->psi, ref, ctx->psi32, p.n_r, ctx->C, p.n_orb, fresh ? 1 : 0);
         // FP64 chunks stay at 16 (32 would need 114 KB of LDS)
         const bool k32 = var == GGB_KIN_TILED32 && mfma_t<T>::mixed;
+        const double2 *kph = ctx->kph;
+        const int32_t *nkeep = ctx->nkeep;
+        const double *psi_in = ctx->psi;
+        double *psi_out = ctx->psi;
+        const float2 *psi32 = ctx->psi32;
+        const int N = p.n_r, nkm = p.nk_max, C = ctx->C, no = p.n_orb;
         if (pass == 0 && k32)
-            k_kin_fwd_tiled<T, 32><<<gf, WG, 0, ctx->stream>>>(Q, ctx->kph, ctx->nkeep, ctx->psi,
-                                                               ctx->psi32, ref, Y, p.n_r, p.nk_max,
-                                                               ctx->C, p.n_orb);
+            kin_launch(k_kin_fwd_tiled<T, 32>, gf, ctx->stream, e0, e1, Q,
+                                  kph, nkeep, psi_in, psi32, ref, Y, N, nkm, C, no);
         else if (pass == 0)
-            k_kin_fwd_tiled<T, 16><<<gf, WG, 0, ctx->stream>>>(Q, ctx->kph, ctx->nkeep, ctx->psi,
-                                                               ctx->psi32, ref, Y, p.n_r, p.nk_max,
-                                                               ctx->C, p.n_orb);
+            kin_launch(k_kin_fwd_tiled<T, 16>, gf, ctx->stream, e0, e1, Q,
+                                  kph, nkeep, psi_in, psi32, ref, Y, N, nkm, C, no);
         else if (k32)
-            k_kin_bwd_tiled<T, 32><<<gb, WG, 0, ctx->stream>>>(Q, ctx->nkeep, Y, ref, ctx->psi,
-                                                               p.n_r, p.nk_max, ctx->C, p.n_orb);
+            kin_launch(k_kin_bwd_tiled<T, 32>, gb, ctx->stream, e0, e1,
+                                  (const T *)Q, nkeep, (const T *)Y, ref, psi_out, N, nkm, C, no);
         else
-            k_kin_bwd_tiled<T, 16><<<gb, WG, 0, ctx->stream>>>(Q, ctx->nkeep, Y, ref, ctx->psi,
-                                                               p.n_r, p.nk_max, ctx->C, p.n_orb);
+            kin_launch(k_kin_bwd_tiled<T, 16>, gb, ctx->stream, e0, e1,
+                                  (const T *)Q, nkeep, (const T *)Y, ref, psi_out, N, nkm, C, no);
         return;
     }
     const int ncol = (2 * ctx->C + 15) / 16;
-    if (pass == 0)
-        k_kin_fwd<T><<<dim3(p.nk_max / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
-            Q, ctx->kph, ctx->nkeep, ctx->psi, ref, Y, p.n_r, p.nk_max, ctx->C, p.n_orb);
-    else
-        k_kin_bwd<T><<<dim3((p.n_r + 15) / 16, ncol, p.n_l), WG, 0, ctx->stream>>>(
-            reinterpret_cast<const T *>(ctx->QT), ctx->nkeep, Y, ref, ctx->psi, p.n_r, p.nk_max,
-            ctx->C, p.n_orb);
+    const double2 *kph = ctx->kph;
+    const int32_t *nkeep = ctx->nkeep;
+    const int N = p.n_r, nkm = p.nk_max, C = ctx->C, no = p.n_orb;
+    if (pass == 0) {
+        const double *psi_in = ctx->psi;
+        kin_launch(k_kin_fwd<T>, dim3(p.nk_max / 16, ncol, p.n_l), ctx->stream, e0, e1, Q, kph, nkeep,
+                   psi_in, ref, Y, N, nkm, C, no);
+    } else {
+        kin_launch(k_kin_bwd<T>, dim3((p.n_r + 15) / 16, ncol, p.n_l), ctx->stream, e0, e1,
+                   reinterpret_cast<const T *>(ctx->QT), nkeep, (const T *)Y, ref, ctx->psi, N, nkm,
+                   C, no);
+    }
 }
 
-static int launch_kinetic_pass(ggb_ctx *ctx, int pass, bool fresh = false) {
+static int launch_kinetic_pass(ggb_ctx *ctx, int pass, bool fresh = false, hipEvent_t e0 = nullptr,
+                               hipEvent_t e1 = nullptr) {
     if (ctx->p.precision)
-        launch_kinetic_t<float>(ctx, pass, fresh);
+        launch_kinetic_t<float>(ctx, pass, fresh, e0, e1);
     else
-        launch_kinetic_t<double>(ctx, pass, fresh);
+        launch_kinetic_t<double>(ctx, pass, fresh, e0, e1);
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
 }
@@ -2123,12 +2148,15 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
             ctx->events.push_back(e);
         }
     }
+    // mode 2: events between all phases; mode 1: the two kinetic product
+    // kernels of every TIMING_STRIDE-th step, stamped by their own dispatch
+    // (start/end of the kernel, events 1-2 and 3-4 of the step)
     auto mark = [&](int s, int k) -> hipError_t {
-        if (!ctx->timing) return hipSuccess;
-        // mode 1: kinetic passes of every TIMING_STRIDE-th step only, so that the
-        // events do not slow the timed run
-        if (ctx->timing == 1 && (k < 1 || k > 3 || s % TIMING_STRIDE)) return hipSuccess;
+        if (ctx->timing != 2) return hipSuccess;
         return hipEventRecord(ctx->events[(size_t)s * ne + k], ctx->stream);
+    };
+    auto kev = [&](int s, int k) -> hipEvent_t {
+        return (ctx->timing == 1 && s % TIMING_STRIDE == 0) ? ctx->events[(size_t)s * ne + k] : nullptr;
     };
     int s_begin = 0;
     if (ctx->graph_steps > 0 && !ctx->timing && nsteps >= ctx->graph_steps) {
@@ -2140,9 +2168,9 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         HIPCHK(ctx, mark(s, 0));
         int rc = launch_first_half(ctx, step);
         HIPCHK(ctx, mark(s, 1));
-        if (!rc) rc = launch_kinetic_pass(ctx, 0, true);
+        if (!rc) rc = launch_kinetic_pass(ctx, 0, true, kev(s, 1), kev(s, 2));
         HIPCHK(ctx, mark(s, 2));
-        if (!rc) rc = launch_kinetic_pass(ctx, 1, true);
+        if (!rc) rc = launch_kinetic_pass(ctx, 1, true, kev(s, 3), kev(s, 4));
         HIPCHK(ctx, mark(s, 3));
         if (!rc) rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3, nullptr,
                                          [&]() { return mark(s, 4); });
@@ -2154,9 +2182,11 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         for (int s = 0; s < nsteps; ++s) {
             for (int k = 0; k < GGB_NPHASE; ++k) {
                 if (ctx->timing == 1 && (k < 1 || k > 2 || s % TIMING_STRIDE)) continue;
+                // mode 1: phase 1 = events 1-2 (pass 1 kernel), phase 2 = events 3-4
+                const int ka = ctx->timing == 1 ? 2 * k - 1 : k;
                 float ms = 0.f;
-                HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->events[(size_t)s * ne + k],
-                                                ctx->events[(size_t)s * ne + k + 1]));
+                HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->events[(size_t)s * ne + ka],
+                                                ctx->events[(size_t)s * ne + ka + 1]));
                 ctx->phase_ms[k] += ms;
                 ctx->phase_n[k] += 1;
             }

@@ -190,11 +190,13 @@ int ggb_autotune(ggb_ctx *ctx, int32_t reps, int32_t *chosen, double *ms);
  * observables, 4 = potential rebuild (density, v_xc, Hartree, phases). */
 #define GGB_NPHASE 5
 
-/* Record HIP events on the context stream around the step phases of later
- * ggb_run calls (mode 1: the two kinetic passes of every 16th step, cheap
- * enough for a timed run; 2: every phase of every step; 0: off) and reset
- * the accumulated times.  No counterpart in the reference, which has no
- * timing code. */
+/* Time the step phases of later ggb_run calls with HIP events on the
+ * context stream and reset the accumulated times.  Mode 1: the product
+ * kernel of each kinetic pass (phases 1 and 2) of every 16th step, with
+ * events stamped by the kernel's own dispatch (hipExtLaunchKernel: kernel
+ * start to end, cheap enough for a timed run); mode 2: events between every
+ * phase of every step; 0: off.  No counterpart in the reference, which has
+ * no timing code. */
 int ggb_set_timing(ggb_ctx *ctx, int32_t mode);
 
 /* Accumulated milliseconds and launch counts per phase [GGB_NPHASE]. */
