@@ -116,21 +116,23 @@ def test_short_run(atom, N, R, lmax, nsteps):
     assert np.abs(psi - ref["psi"]).max() < 1e-10
 
 
-def test_batched_instances_match_alone():
+@pytest.mark.parametrize("n_inst", [3, 5])
+def test_batched_instances_match_alone(n_inst):
     """Instances with different fields in one context give the same results
     as each instance alone (per-column arithmetic does not depend on the mix)."""
-    _say("case: batched vs alone, Ne N=48 Lmax=6, 3 instances")
+    _say("case: batched vs alone, Ne N=48 Lmax=6, %d instances" % n_inst)
     dt = 0.05
     S = AtomSystem("Ne", 48, 30.0, 6, dt)
     nsteps = 25
     t = (np.arange(nsteps) + 0.5) * dt
-    fields = np.stack([0.05 * np.sin(0.4 * t), -0.08 * np.cos(0.3 * t), 0.02 + 0.0 * t])
+    fields = np.stack([0.05 * np.sin(0.4 * t), -0.08 * np.cos(0.3 * t), 0.02 + 0.0 * t,
+                       0.06 * np.sin(0.2 * t + 1.0), -0.03 + 0.0 * t])[:n_inst]
     eng = _engine()
-    with eng.Propagator(S, n_inst=3) as prop:
+    with eng.Propagator(S, n_inst=n_inst) as prop:
         prop.set_field(fields)
         mix = prop.run(0, nsteps)
         psi_mix = prop.download_orbitals()
-    for k in range(3):
+    for k in range(n_inst):
         with eng.Propagator(S, n_inst=1) as prop:
             prop.set_field(fields[k])
             alone = prop.run(0, nsteps)
