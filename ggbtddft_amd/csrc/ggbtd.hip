@@ -598,6 +598,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
 // max(ceil(h/16), ceil(ns/16) + ceil(na/16)) <= 8 waves.
 // ---------------------------------------------------------------------------
 #define ANG_MAXW 8
+#define ANG_MAX_NL 128  // largest N_l ggb_create accepts
 struct AngArgs {
     double *psi;
     const void *Bh;        // [n_m][nblk] parity half tables, compute type
@@ -613,24 +614,26 @@ struct AngArgs {
     float2 *psi32;         // mode 0, FP32 tiled kinetic: FP32 copy of the stored state
     const int32_t *l0;     // blocks left to k_kin_prep (ground-state component), or null
     int N, Nl, Nlx, Nth, h, hq, sS, sA, nblk, C, n_orb, n_field, step;
+    int alias;  // mixed form: S/D live in the l-space tile's LDS (see k_angular)
     double dt;
 };
 
 static __device__ __forceinline__ double c_lm(int l, int m) {
-    // <l m|cos theta|l+1 m> (internal.py:80-84, angularCoefficients.cuh d_c); computed
-    // in registers rather than read from a table: the laser sits on the latency path
+    // <l m|cos theta|l+1 m> (internal.py:80-84, angularCoefficients.cuh d_c)
     const double dl = (double)l, dm = (double)m;
     return sqrt(((dl + 1.0) * (dl + 1.0) - dm * dm) / ((2.0 * dl + 1.0) * (2.0 * dl + 3.0)));
 }
 
-static __device__ void ang_laser(double *X, int parity, int m, int Nl, const double *a_i) {
+// cl[l] = c_lm(l, m) of the work-group's m, staged in LDS once
+static __device__ void ang_laser(double *X, int parity, int m, int Nl, const double *a_i,
+                                 const double *cl) {
     // pairs (l, l+1), l = m + parity + 2 q, l + 1 < Nl; thread -> (q, i_local)
     const int t = threadIdx.x;
     const int il = t & 7;
     for (int q = t >> 3;; q += blockDim.x / 8) {
         const int l = m + parity + 2 * q;
         if (l + 1 >= Nl) break;
-        const double th = a_i[il] * c_lm(l, m);
+        const double th = a_i[il] * cl[l];
         double sn, cs;
         sincos(th, &sn, &cs);
         double *x0 = X + l * 16 + 2 * il;
@@ -680,11 +683,23 @@ static __device__ void ang_fwd(const T *Bs, const T *Ba, const double *X, const 
     o = ao;
 }
 
-// back product of output tile `item` (s tiles, then a tiles) into X; two
-// accumulator chains over alternating groups of 4 nodes, added at the end.
+// row of X that output r of this lane's back-product item `item` (s tiles,
+// then a tiles) belongs to, or -1 past the parity's last l
 template <typename T>
-static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D, double *X,
-                               const AngArgs &g, const ang_par &pr, int m, int item) {
+static __device__ int ang_bwd_row(const ang_par &pr, int m, int item, int r) {
+    const int kr = (threadIdx.x & 63) >> 4;
+    const int nst = (pr.ns + 15) / 16;
+    const bool sym = item < nst;
+    const int ko = (sym ? item : item - nst) * 16 + mfma_t<T>::row(kr, r);
+    return ko < (sym ? pr.ns : pr.na) ? m + (sym ? 0 : 1) + 2 * ko : -1;
+}
+
+// back product of output tile `item` (s tiles, then a tiles): v[r] for the
+// rows ang_bwd_row gives; two accumulator chains over alternating groups of
+// 4 nodes, added at the end.
+template <typename T>
+static __device__ void ang_bwd_sum(const T *Bs, const T *Ba, const T *S, const T *D,
+                                   const AngArgs &g, const ang_par &pr, int item, double v[4]) {
     typedef typename mfma_t<T>::v4 v4;
     const int lane = threadIdx.x & 63, kr = lane >> 4, lc = lane & 15;
     const int nst = (pr.ns + 15) / 16;
@@ -694,7 +709,6 @@ static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D,
     const T *Pv = sym ? S : D;
     const int st = sym ? g.sS : g.sA;
     const int kk = sym ? pr.ks : pr.ka;
-    const int nn = sym ? pr.ns : pr.na;
     // output rows k >= kk read column kk - 1 (in range); they are not stored
     const T *Ac = A + min(tile * 16 + lc, kk - 1);
     v4 a0 = (v4){0, 0, 0, 0}, a1 = (v4){0, 0, 0, 0};
@@ -708,14 +722,23 @@ static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D,
         const int ja = j0 + kr;
         a0 = mfma_t<T>::op(Ac[(size_t)ja * st], Pv[ja * 16 + lc], a0);
     }
-    const int off = sym ? 0 : 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = (double)(a0[r] + a1[r]);
+}
+
+// back product into X (mixed form: added to the values there)
+template <typename T>
+static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D, double *X,
+                               const AngArgs &g, const ang_par &pr, int m, int item) {
+    double v[4];
+    ang_bwd_sum<T>(Bs, Ba, S, D, g, pr, item, v);
+    const int lc = threadIdx.x & 15;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int ko = tile * 16 + mfma_t<T>::row(kr, r);
-        if (ko < nn) {
-            double &x = X[(m + off + 2 * ko) * 16 + lc];
-            const double v = (double)(a0[r] + a1[r]);
-            x = mfma_t<T>::mixed ? x + v : v;
+        const int l = ang_bwd_row<T>(pr, m, item, r);
+        if (l >= 0) {
+            double &x = X[l * 16 + lc];
+            x = mfma_t<T>::mixed ? x + v[r] : v[r];
         }
     }
 }
@@ -725,6 +748,7 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     typedef typename mfma_t<T>::v4 v4;
     extern __shared__ __align__(16) double lds[];
     __shared__ double a_i[RT];
+    __shared__ double cl[ANG_MAX_NL];
     __shared__ double red[3][ANG_MAXW];
 
     // logical order (orbital, radial tile, instance), orbital fastest: the
@@ -746,7 +770,8 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     const T *Bsrc = reinterpret_cast<const T *>(g.Bh) + (size_t)m * g.nblk;
     T *Tl = reinterpret_cast<T *>(lds);
     double *X = reinterpret_cast<double *>(GLB ? Tl : Tl + g.nblk);  // [Nlx][16]
-    T *S = reinterpret_cast<T *>(X + g.Nlx * 16);                    // [hq][16]
+    const bool AL = mfma_t<T>::mixed && g.alias;                     // S/D inside X
+    T *S = reinterpret_cast<T *>(AL ? X : X + g.Nlx * 16);           // [hq][16]
     T *D = S + g.hq * 16;                                            // [hq][16]
     const T *Bs = GLB ? Bsrc : Tl;
     const T *Ba = Bs + (size_t)g.hq * g.sS;
@@ -776,17 +801,24 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         const double Ev = g.E[(size_t)inst * g.n_field + fs];
         a_i[t] = (i < g.N) ? 0.5 * g.dt * Ev * g.r[i] : 0.0;
     }
+    if (MODE != 2)
+        for (int l = t; l < g.Nl; l += nthr) cl[l] = c_lm(l, m);
     __syncthreads();
 
     if (MODE == 1) {
-        ang_laser(X, 1, m, g.Nl, a_i);
+        ang_laser(X, 1, m, g.Nl, a_i, cl);
         __syncthreads();
-        ang_laser(X, 0, m, g.Nl, a_i);
+        ang_laser(X, 0, m, g.Nl, a_i, cl);
         __syncthreads();
     }
 
     if (MODE != 2) {
-        // Vh: theta space, phase, folded sums S/D, back to l space
+        // Vh: theta space, phase, folded sums S/D, back to l space.  With S/D
+        // inside the l-space tile (AL): the forward products and this wave's
+        // old tile values are taken first, S/D written after a barrier, the
+        // back products formed in registers, and the tile rewritten after a
+        // barrier (rows l < m and l >= N_l back to zero)
+        T svr[4], dvr[4];
         if (wave < mt) {
             v4 e, o;
             ang_fwd<T>(Bs, Ba, X, g, pr, m, wave, e, o);
@@ -809,28 +841,69 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                         const double2 fa_ = ph[j], fb_ = ph[jm];
                         fax = (T)fa_.x, fay = (T)fa_.y, fbx = (T)fb_.x, fby = (T)fb_.y;
                     }
-                    const T ra = p ? pao : pa, ia = p ? pa : pao;
-                    const T rb = p ? pbo : pb, ib = p ? pb : pbo;
-                    const T qa = p ? (ra * fay + ia * fax) : (ra * fax - ia * fay);
-                    const T qb = p ? (rb * fby + ib * fbx) : (rb * fbx - ib * fby);
+                    // component p of f (x + i y): x fx - y fy (p = 0), y fx + x fy (p = 1);
+                    // this lane holds component p, its partner the other
+                    const T sg = p ? (T)1 : (T)-1;
+                    const T qa = pa * fax + sg * pao * fay;
+                    const T qb = pb * fbx + sg * pbo * fby;
                     sv = (jm == j) ? qa : qa + qb;
                     dv = (jm == j) ? qa : qa - qb;
                 }
-                if (j < g.hq) {
+                svr[r] = sv;
+                dvr[r] = dv;
+                if (!AL && j < g.hq) {
                     S[j * 16 + lc] = sv;
                     D[j * 16 + lc] = dv;
                 }
             }
         }
-        __syncthreads();
-        if (wave < nbt) ang_bwd<T>(Bs, Ba, S, D, X, g, pr, m, wave);
+        if (AL) {
+            double xo[4];
+            if (wave < nbt) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int l = ang_bwd_row<T>(pr, m, wave, r);
+                    xo[r] = l >= 0 ? X[l * 16 + lc] : 0.0;
+                }
+            }
+            __syncthreads();
+            if (wave < mt) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = wave * 16 + mfma_t<T>::row(kr, r);
+                    if (j < g.hq) {
+                        S[j * 16 + lc] = svr[r];
+                        D[j * 16 + lc] = dvr[r];
+                    }
+                }
+            }
+            __syncthreads();
+            double v[4];
+            if (wave < nbt) ang_bwd_sum<T>(Bs, Ba, S, D, g, pr, wave, v);
+            __syncthreads();
+            const int npad = g.Nlx - g.Nl;
+            for (int idx = t; idx < (m + npad) * 16; idx += nthr) {
+                const int row = idx / 16 < m ? idx / 16 : g.Nl + idx / 16 - m;
+                X[row * 16 + idx % 16] = 0.0;
+            }
+            if (wave < nbt) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int l = ang_bwd_row<T>(pr, m, wave, r);
+                    if (l >= 0) X[l * 16 + lc] = xo[r] + v[r];
+                }
+            }
+        } else {
+            __syncthreads();
+            if (wave < nbt) ang_bwd<T>(Bs, Ba, S, D, X, g, pr, m, wave);
+        }
         __syncthreads();
     }
 
     if (MODE == 0) {
-        ang_laser(X, 0, m, g.Nl, a_i);
+        ang_laser(X, 0, m, g.Nl, a_i, cl);
         __syncthreads();
-        ang_laser(X, 1, m, g.Nl, a_i);
+        ang_laser(X, 1, m, g.Nl, a_i, cl);
         __syncthreads();
     }
 
@@ -851,6 +924,7 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
 
     // |B psi|^2 and observable partials (accumulated in FP64)
     const double ri = iok ? g.r[i] : 1.0;
+    const double ir2 = 1.0 / (ri * ri);
     double sd = 0.0, sa = 0.0, sn = 0.0;
     if (wave < mt) {
         v4 e, o;
@@ -871,7 +945,7 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                     const double ca = g.chi[j];
                     dens[jm] = (T)s2b;
                     sd += ri * ca * eo;
-                    sa += ca * eo / (ri * ri);
+                    sa += ca * eo * ir2;
                     sn += s2a + s2b;
                 } else {
                     sn += s2a;  // middle node, chi = 0
@@ -1344,11 +1418,12 @@ __global__ void k_selftest_mfma(double *out) {
 // Host side
 // ===========================================================================
 // Sizes of the parity-split angular tables and the angular kernel's launch.
-#define ANG_LDS_MAX 65536  // tables staged in LDS when everything fits in 64 KiB
+#define ANG_LDS_MAX 81920  // tables staged in LDS up to 80 KiB (two work-groups per CU on gfx950)
 struct ang_geom {
     int h, hq, sS, sA, nblk, waves;
     size_t lds;  // dynamic LDS bytes
     bool glb;    // tables read from global memory
+    bool alias;  // mixed form: S/D share the l-space tile's LDS
 };
 static ang_geom ang_geometry(const ggb_problem *p) {
     ang_geom a;
@@ -1363,8 +1438,14 @@ static ang_geom ang_geometry(const ggb_problem *p) {
     a.nblk = ((a.hq * (a.sS + a.sA) + per16 - 1) / per16) * per16;
     const int mt = (a.h + 15) / 16, nb = (ns0 + 15) / 16 + (na0 + 15) / 16;
     a.waves = mt > nb ? mt : nb;
-    const size_t tiles = (size_t)(p->n_l + 6) * 16 * sizeof(double) + (size_t)2 * a.hq * 16 * es;
-    const size_t stat = (RT + 3 * ANG_MAXW) * sizeof(double);
+    // mixed form: the folded sums S/D (FP32) fit in the FP64 l-space tile's
+    // space whenever hq <= N_l + 6; the kernel parks the old tile values it
+    // needs in registers meanwhile (one work-group less of LDS per CU)
+    const size_t xbytes = (size_t)(p->n_l + 6) * 16 * sizeof(double);
+    const size_t sdbytes = (size_t)2 * a.hq * 16 * es;
+    a.alias = p->precision && sdbytes <= xbytes;
+    const size_t tiles = a.alias ? xbytes : xbytes + sdbytes;
+    const size_t stat = (RT + ANG_MAX_NL + 3 * ANG_MAXW) * sizeof(double);
 #ifndef ANG_FORCE_GLB
 #define ANG_FORCE_GLB 0
 #endif
@@ -1760,6 +1841,7 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
     g.sS = ctx->ang.sS;
     g.sA = ctx->ang.sA;
     g.nblk = ctx->ang.nblk;
+    g.alias = ctx->ang.alias ? 1 : 0;
     g.C = ctx->C;
     g.n_orb = ctx->p.n_orb;
     g.n_field = ctx->n_field;
