@@ -585,7 +585,8 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
 // (hq x (sS + sA) values) are staged in LDS once per work-group (GLB = true
 // reads them from global memory when they do not fit).
 // LDS: tables (T) | X[Nl][16] (FP64) | S[hq][16] | D[hq][16] (T), real column
-// n = 2 * i_local + re/im.  The state tile X, the laser rotations, the phase
+// n = 2 * i_local + re/im; in the mixed form S/D overlay X during the
+// potential half step (AngArgs.alias).  The state tile X, the laser rotations, the phase
 // factors and the observable sums are FP64 in both forms; the mixed form
 // applies the potential half step as X += B^T ((f - 1) B X).  The dipole-type
 // sums use sum_j chi_j |Psi_j|^2 = sum_{j<h} chi_j 4 Re(E_j^* O_j), so that the
