@@ -1131,7 +1131,7 @@ struct QmomEpi {
 // scans (ratio form, no overflow):
 //   A_a = A_{a-1} (r_{a-1}/r_a)^{L+1} + q_a / r_a
 //   B_a = (r_a/r_{a+1})^L (B_{a+1} + q_{a+1} / r_{a+1}),   V = A + B
-// with the ratio powers as exp(n lr), lr[a] = log(r[a-1]/r[a]) from the host.
+// with the ratio powers tabulated on the host as exp(n log(r_{a-1}/r_a)).
 // One work-group per (L, instance): each thread composes the affine maps of
 // a contiguous chunk of radial nodes; the chunk maps are scanned within each
 // wave by shuffles and across the four waves through LDS (one barrier), in a
@@ -1165,9 +1165,11 @@ static __device__ void affine_scan(double &al, double &be, int dir, double (*sm)
     __syncthreads();
 }
 
+// hp: the ratio powers and 1/r, from the host (shared by all instances):
+// hp[L][a] = (r_{a-1}/r_a)^{L+1} (0 at a = 0), hp[n_har + L][a] = (r_a/r_{a+1})^L
+// (a < N - 1), hp[2 n_har][a] = 1/r_a
 __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
-                                                const double *__restrict__ r,
-                                                const double *__restrict__ lr,
+                                                const double *__restrict__ hp,
                                                 double *__restrict__ V, int N, int n_har) {
     __shared__ double sm[WG / 64][2];
     __shared__ double edge[2][WG];
@@ -1175,14 +1177,16 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
     const int t = threadIdx.x;
     const double *qL = q + ((size_t)inst * n_har + L) * N;
     double *VL = V + ((size_t)inst * n_har + L) * N;
+    const double *alpha = hp + (size_t)L * N;
+    const double *gamma = hp + ((size_t)n_har + L) * N;
+    const double *ir = hp + (size_t)2 * n_har * N;
     const int cs = (N + WG - 1) / WG;
     const int s0 = min(N, t * cs), s1 = min(N, s0 + cs);
     // ---- prefix: A_a = alpha_a A_{a-1} + beta_a
     double al = 1.0, be = 0.0;
     for (int a = s0; a < s1; ++a) {
-        const double alpha = (a > 0) ? exp((double)(L + 1) * lr[a]) : 0.0;
-        al *= alpha;
-        be = be * alpha + qL[a] / r[a];
+        al *= alpha[a];
+        be = be * alpha[a] + qL[a] * ir[a];
     }
     affine_scan(al, be, 1, sm);
     edge[0][t] = be;  // A at the end of this chunk
@@ -1194,17 +1198,16 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
             sb = 0.0;
             continue;
         }
-        const double gam = exp((double)L * lr[a + 1]);
+        const double gam = gamma[a];
         sa *= gam;
-        sb = sb * gam + gam * qL[a + 1] / r[a + 1];
+        sb = sb * gam + gam * qL[a + 1] * ir[a + 1];
     }
     affine_scan(sa, sb, -1, sm);
     edge[1][t] = sb;  // B at the start of this chunk
     __syncthreads();
     double A = t > 0 ? edge[0][t - 1] : 0.0;
     for (int a = s0; a < s1; ++a) {
-        const double alpha = (a > 0) ? exp((double)(L + 1) * lr[a]) : 0.0;
-        A = A * alpha + qL[a] / r[a];
+        A = A * alpha[a] + qL[a] * ir[a];
         VL[a] = A;
     }
     double Bv = t < WG - 1 ? edge[1][t + 1] : 0.0;
@@ -1212,8 +1215,8 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
         if (a == N - 1) {
             Bv = 0.0;
         } else {
-            const double gam = exp((double)L * lr[a + 1]);
-            Bv = gam * (Bv + qL[a + 1] / r[a + 1]);
+            const double gam = gamma[a];
+            Bv = gam * (Bv + qL[a + 1] * ir[a + 1]);
         }
         VL[a] += Bv;
     }
@@ -1465,7 +1468,7 @@ struct ggb_ctx {
     int n_field;
     bool field_ring;  // ggb_field_ring: field index = step mod n_field
     // device buffers
-    double *psi, *Y, *Q, *QT, *B, *PL, *PLT, *chi, *omega, *r, *lr, *wr, *v_gs, *mask, *occ, *rho_frozen;
+    double *psi, *Y, *Q, *QT, *B, *PL, *PLT, *chi, *omega, *r, *hpow, *wr, *v_gs, *mask, *occ, *rho_frozen;
     int32_t *nkeep, *mcol;
     double2 *kph;  // [n_l][nk_max] (cos, sin)(lam dt) of the kinetic propagator
     double *dens, *vxc, *rho_u, *q, *V, *dv, *obs_part, *obs, *E;
@@ -1532,7 +1535,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
     hipSetDevice(ctx->p.device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->QT, ctx->kph, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
-                    ctx->r, ctx->lr, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
+                    ctx->r, ctx->hpow, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
                     ctx->l0, ctx->ab, ctx->psi32};
@@ -1555,7 +1558,8 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     size_t tbytes = (Nl * C * nk * 2 + 2 * Nl * N * nk + (size_t)p->n_m * nblk) * es;  // Y, Q, QT, B
     d += 2 * Nl * nk;                    // e^{-i lam dt}
     d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
-    d += 6 * N + p->n_orb;               // r, lr, wr, v_gs, mask, rho_frozen, occ
+    d += 5 * N + p->n_orb;               // r, wr, v_gs, mask, rho_frozen, occ
+    d += (2 * nh + 1) * N;               // Hartree ratio powers and 1/r
     d += C * N * Nth;                    // dens
     d += 3 * nI * N * Nth;               // vxc, rho_u, dv
     d += 2 * nI * N * Nth;               // phase (double2)
@@ -1606,7 +1610,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->obs_cap = 0;
     ctx->kph = nullptr;
     ctx->psi = ctx->Y = ctx->Q = ctx->QT = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
-    ctx->r = ctx->lr = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
+    ctx->r = ctx->hpow = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
     ctx->nkeep = ctx->mcol = nullptr;
     ctx->dens = ctx->vxc = ctx->rho_u = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
     ctx->phase = nullptr;
@@ -1652,7 +1656,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ok &= dalloc(&ctx->chi, Nth) == hipSuccess;
     ok &= dalloc(&ctx->omega, Nth) == hipSuccess;
     ok &= dalloc(&ctx->r, N) == hipSuccess;
-    ok &= dalloc(&ctx->lr, N) == hipSuccess;
+    ok &= dalloc(&ctx->hpow, (2 * (size_t)p->n_har + 1) * N) == hipSuccess;  // Hartree ratio powers, 1/r
     ok &= dalloc(&ctx->wr, N) == hipSuccess;
     ok &= dalloc(&ctx->v_gs, N) == hipSuccess;
     ok &= dalloc(&ctx->mask, N) == hipSuccess;
@@ -1707,10 +1711,18 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
     hipStream_t st = ctx->stream;
     HIPCHK(ctx, hipMemcpyAsync(ctx->r, s->r, N * 8, hipMemcpyHostToDevice, st));
     {
-        // lr[a] = log(r[a-1]/r[a]): ratio powers of the Hartree scans as exp(n lr)
-        std::vector<double> lrh(N, 0.0);
+        // the Hartree scans' ratio powers as exp(n log(r_{a-1}/r_a)) (no overflow
+        // at large L) and 1/r, laid out as k_hartree reads them
+        const size_t nh = p.n_har;
+        std::vector<double> lrh(N, 0.0), hp((2 * nh + 1) * N, 0.0);
         for (size_t a = 1; a < N; ++a) lrh[a] = std::log(s->r[a - 1] / s->r[a]);
-        HIPCHK(ctx, hipMemcpy(ctx->lr, lrh.data(), N * 8, hipMemcpyHostToDevice));
+        for (size_t L = 0; L < nh; ++L)
+            for (size_t a = 0; a < N; ++a) {
+                hp[L * N + a] = a > 0 ? std::exp((double)(L + 1) * lrh[a]) : 0.0;
+                hp[(nh + L) * N + a] = a + 1 < N ? std::exp((double)L * lrh[a + 1]) : 0.0;
+            }
+        for (size_t a = 0; a < N; ++a) hp[2 * nh * N + a] = 1.0 / s->r[a];
+        HIPCHK(ctx, hipMemcpy(ctx->hpow, hp.data(), hp.size() * 8, hipMemcpyHostToDevice));
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->wr, s->wr, N * 8, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->v_gs, s->v_gs, N * 8, hipMemcpyHostToDevice, st));
@@ -1915,7 +1927,7 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
         gq.K = Nth;
         QmomEpi eq{ctx->q, N, nh};
         k_tile_gemm<<<dim3((N + 15) / 16, (nh + 15) / 16, p.n_inst), WG, 0, st>>>(gq, eq);
-        k_hartree<<<dim3(nh, p.n_inst), WG, 0, st>>>(ctx->q, ctx->r, ctx->lr, ctx->V, N, nh);
+        k_hartree<<<dim3(nh, p.n_inst), WG, 0, st>>>(ctx->q, ctx->hpow, ctx->V, N, nh);
     }
     GemmOperands gv;  // V(i, theta_j) = sum_L V[L][i] PLT[L][j]
     gv.A = ctx->V;
