@@ -187,46 +187,58 @@ def measured_traffic():
         return None
 
 
-def precision_report(sysm, pulse):
-    """Full-length runs in FP64 and FP32; errors of the FP32 spectrum."""
+def precision_report(sysm, pulse, n_batch=16):
+    """Full-length runs: FP64 one instance, FP32 mixed one instance (narrow
+    kinetic kernels) and FP32 mixed in a batch of n_batch instances (tiled
+    kinetic kernels, the configuration of the batched leg; instance 0 carries
+    the benchmark pulse, the others 1e14-3e14 W/cm^2); errors of the FP32
+    spectra against FP64."""
     from ggbtddft_amd import spectrum as sp
     from ggbtddft_amd.engine import Propagator
+    from ggbtddft_amd.pulse import Pulse
     n = pulse.n_steps(DT)
     E = pulse.midpoints(DT)
     E_end = pulse((np.arange(n) + 1.0) * DT)
-    out = {}
-    for prec in (0, 1):
+    rng = np.random.default_rng(1)
+    fields = [E] + [Pulse(float(I), WAVELENGTH, CYCLES).midpoints(DT)[:n]
+                    for I in rng.uniform(1e14, 3e14, n_batch - 1)]
+    runs = {}
+    for name, prec, ninst in (("fp64", 0, 1), ("fp32", 1, 1), ("fp32_batched", 1, n_batch)):
         t0 = time.perf_counter()
-        with Propagator(sysm, precision=prec) as prop:
-            prop.set_field(E)
+        with Propagator(sysm, n_inst=ninst, precision=prec) as prop:
+            prop.set_field(np.stack(fields[:ninst]) if ninst > 1 else E)
+            if ninst > 1:
+                prop.autotune(reps=3)
             o = prop.run(0, n)
-        out[prec] = (sysm.total_acceleration(o["acc"][:, 0, :], o["norm"][:, 0, :], E_end),
-                     time.perf_counter() - t0, o["dip"][:, 0, :] @ sysm.occ)
-        log("precision %d full run: %d steps %.1f s" % (prec, n, out[prec][1]))
-    a64, a32 = out[0][0], out[1][0]
-    rel = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+        runs[name] = (sysm.total_acceleration(o["acc"][:, 0, :], o["norm"][:, 0, :], E_end),
+                      o["dip"][:, 0, :] @ sysm.occ, time.perf_counter() - t0)
+        log("precision %s full run: %d steps, %d instance(s), %.1f s" % (name, n, ninst, runs[name][2]))
+    a64, d64 = runs["fp64"][0], runs["fp64"][1]
     w, S64 = sp.acceleration_spectrum(a64, DT)
-    _, S32 = sp.acceleration_spectrum(a32, DT)
+    wd, D64 = sp.dipole_spectrum(d64, DT)
     ip = -float(sysm.gs.eps.max())
     cut = sp.cutoff_energy(pulse.E0, pulse.omega0, ip) / pulse.omega0
     orders = np.arange(3, int(cut) + 8, 2)
-    I64 = sp.harmonic_intensities(w, S64, pulse.omega0, orders)
-    I32 = sp.harmonic_intensities(w, S32, pulse.omega0, orders)
-    err = np.log10(I32 / I64)
     plateau = orders < 0.8 * cut
-    d64, d32 = out[0][2], out[1][2]
-    wd, D64 = sp.dipole_spectrum(d64, DT)
-    _, D32 = sp.dipole_spectrum(d32, DT)
-    errd = np.log10(sp.harmonic_intensities(wd, D32, pulse.omega0, orders) /
-                    sp.harmonic_intensities(wd, D64, pulse.omega0, orders))
-    return {"steps": n, "acc_rel_l2": rel, "cutoff_order": cut,
-            "dipole_rel_l2": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
-            "dipole_form_log10_error": errd.tolist(),
-            "orders": orders.tolist(), "log10_intensity_error": err.tolist(),
-            "max_abs_log10_err_plateau": float(np.abs(err[plateau]).max()),
-            "max_abs_log10_err_cutoff": float(np.abs(err[~plateau]).max()),
-            "one_percent_in_log10": float(np.log10(1.01)),
-            "wall_s_fp64": out[0][1], "wall_s_fp32": out[1][1]}
+    I64 = sp.harmonic_intensities(w, S64, pulse.omega0, orders)
+    J64 = sp.harmonic_intensities(wd, D64, pulse.omega0, orders)
+    rep = {"steps": n, "cutoff_order": cut, "orders": orders.tolist(),
+           "one_percent_in_log10": float(np.log10(1.01)), "wall_s_fp64": runs["fp64"][2],
+           "batch_instances": n_batch}
+    for name in ("fp32", "fp32_batched"):
+        a32, d32, wall = runs[name]
+        _, S32 = sp.acceleration_spectrum(a32, DT)
+        _, D32 = sp.dipole_spectrum(d32, DT)
+        err = np.log10(sp.harmonic_intensities(w, S32, pulse.omega0, orders) / I64)
+        errd = np.log10(sp.harmonic_intensities(wd, D32, pulse.omega0, orders) / J64)
+        rep[name] = {"acc_rel_l2": float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64)),
+                     "dipole_rel_l2": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
+                     "log10_intensity_error": err.tolist(),
+                     "dipole_form_log10_error": errd.tolist(),
+                     "max_abs_log10_err_plateau": float(np.abs(err[plateau]).max()),
+                     "max_abs_log10_err_cutoff": float(np.abs(err[~plateau]).max()),
+                     "wall_s": wall}
+    return rep
 
 
 def main():
