@@ -35,30 +35,50 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(system_args, E, nsteps=40, warm=3):
-    """FP64 oracle on the host for a bounded window (setup excluded)."""
+def cpu_baseline(system_args, E, nsteps=40, warm=3, nsteps_1core=10):
+    """FP64 oracle on the host for a bounded window (setup excluded): all
+    cores (BLAS threads = min(16, nproc)), then one core on a shorter window."""
     import contextlib
     from oracle import propagate as OP
+
+    def limited(n):
+        try:
+            from threadpoolctl import threadpool_limits
+            return threadpool_limits(limits=n)
+        except Exception:  # pragma: no cover
+            return contextlib.nullcontext()
+
     cores = min(16, os.cpu_count() or 1)
-    try:
-        from threadpoolctl import threadpool_limits
-        limit = threadpool_limits(limits=cores)
-    except Exception:  # pragma: no cover
-        limit = contextlib.nullcontext()
-    with limit:
-        t0 = time.perf_counter()
-        S = OP.System(*system_args)
-        setup = time.perf_counter() - t0
-        psi = S.psi0.copy()
-        res = S.run(E[:warm], psi=psi)
-        t0 = time.perf_counter()
-        S.run(E[warm:warm + nsteps], psi=res["psi"])
-        el = time.perf_counter() - t0
-    log("cpu baseline: %d steps in %.1f s" % (nsteps, el))
-    return dict(value=S.norb * nsteps / el, unit="orbital-timesteps/s", cores=cores,
-                kind="port",
+    t0 = time.perf_counter()
+    S = OP.System(*system_args)
+    setup = time.perf_counter() - t0
+    rates = {}
+    for n, k in ((cores, nsteps), (1, nsteps_1core)):
+        with limited(n):
+            res = S.run(E[:warm], psi=S.psi0.copy())
+            t0 = time.perf_counter()
+            S.run(E[warm:warm + k], psi=res["psi"])
+            el = time.perf_counter() - t0
+        rates[n] = (S.norb * k / el, k, el)
+        log("cpu baseline: %d steps on %d core(s) in %.1f s" % (k, n, el))
+    v, k, el = rates[cores]
+    v1, k1, el1 = rates[1]
+    return dict(value=v, unit="orbital-timesteps/s", cores=cores, kind="port",
                 sample="oracle (numpy FP64), Ar N=512 Lmax=64, %d steps after %d warm-up "
-                       "steps, %.1f s (setup %.1f s excluded)" % (nsteps, warm, el, setup))
+                       "steps, %.1f s (setup %.1f s excluded)" % (k, warm, el, setup),
+                value_1core=v1,
+                sample_1core="same, one thread, %d steps, %.1f s" % (k1, el1),
+                host_cpu=_cpu_model())
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def batched_run(sysm, pulse, args, fp64_value):
