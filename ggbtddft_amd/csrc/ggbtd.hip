@@ -2589,12 +2589,13 @@ extern "C" int ggb_selftest_copy(int32_t device, uint64_t bytes, double *ms) {
 // is, so the fixed point is the one of the exact iteration; the spectral
 // preconditioner (positive definite) only sets the rate.  All FP64.
 // ---------------------------------------------------------------------------
-#define GS_WG 512
-#define GS_MAX_N 1536   // k_gs_sweep keeps 5 N-vectors in LDS (60 KiB)
+#define GS_WG 1024
+#define GS_MAX_N 1536   // k_gs_sweep keeps 5 N-vectors in LDS (60 KiB) plus 8 KiB of scratch
 #define GS_MAX_K 8      // shells per l group
 
 struct GsArgs {
-    const double *r, *wr, *T, *Q0, *Q0T, *lam0, *occ;
+    const double *r, *wr, *T, *lam0, *occ;
+    const float *Q0, *Q0T;  // bare-nucleus eigenvectors [g][i][k] and transposes (preconditioner)
     const int32_t *gsh;  // [n_group][GS_MAX_K] shells of each l group, shell order, -1 padded
     const int32_t *gl;   // [n_group] l of each group
     double *u;           // [n_shell][N]
@@ -2685,56 +2686,67 @@ __global__ __launch_bounds__(GS_WG) void k_gs_potential(GsArgs a) {
     if (t == 0) a.res[0] = dmax;
 }
 
-// y = H x for the group's l (T symmetric: thread i reads column i, coalesced)
-static __device__ void gs_apply_h(const GsArgs &a, int l, const double *x, double *y) {
-    const int N = a.N;
-    const double cl = 0.5 * (double)l * (double)(l + 1);
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        const double *Tc = a.T + i;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int j = 0;
-        for (; j + 4 <= N; j += 4) {
-            s0 += Tc[(size_t)j * N] * x[j];
-            s1 += Tc[(size_t)(j + 1) * N] * x[j + 1];
-            s2 += Tc[(size_t)(j + 2) * N] * x[j + 2];
-            s3 += Tc[(size_t)(j + 3) * N] * x[j + 3];
+// out[c] = sum_j M[j][c] x[j] (M row-major N x N; threads read along c,
+// coalesced).  With N < blockDim the j range is split into P = blockDim / N
+// parts (more loads in flight), the parts added in fixed order through scr
+// ((P - 1) N <= blockDim doubles); eight partial sums per thread.
+template <typename TE>
+static __device__ void gs_colsum(const TE *M, const double *x, int N, double *out, double *scr) {
+    const int nt = blockDim.x;
+    const int P = N < nt ? nt / N : 1;
+    const int jl = (N + P - 1) / P;
+    for (int idx = threadIdx.x; idx < P * N; idx += nt) {
+        const int c = idx % N, part = idx / N;
+        const int j0 = part * jl, j1 = min(N, j0 + jl);
+        const TE *Mc = M + c;
+        double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int j = j0;
+        for (; j + 8 <= j1; j += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s[u] += (double)Mc[(size_t)(j + u) * N] * x[j + u];
         }
-        for (; j < N; ++j) s0 += Tc[(size_t)j * N] * x[j];
-        const double ri = a.r[i];
-        const double d = cl / (ri * ri) - a.Z / ri + a.v[i];
-        y[i] = -0.5 * ((s0 + s1) + (s2 + s3)) + d * x[i];
+        for (; j < j1; ++j) s[0] += (double)Mc[(size_t)j * N] * x[j];
+        const double v = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+        if (part == 0)
+            out[c] = v;
+        else
+            scr[(size_t)(part - 1) * N + c] = v;
+    }
+    __syncthreads();
+    if (P > 1) {
+        for (int c = threadIdx.x; c < N; c += nt) {
+            double v = out[c];
+            for (int q = 1; q < P; ++q) v += scr[(size_t)(q - 1) * N + c];
+            out[c] = v;
+        }
+        __syncthreads();
     }
 }
 
-// w = Q0 diag(1/(|lam0 - theta| + gamma)) Q0^T r, through tmp (all in LDS)
-static __device__ void gs_precondition(const GsArgs &a, int g, double theta, const double *rv,
-                                       double *tmp, double *w) {
+// y = H x for the group's l (T symmetric, read by columns)
+static __device__ void gs_apply_h(const GsArgs &a, int l, const double *x, double *y, double *scr) {
     const int N = a.N;
-    const double *Q = a.Q0 + (size_t)g * N * N;    // [i][k]
-    const double *QT = a.Q0T + (size_t)g * N * N;  // [k][i]
-    const double *lam = a.lam0 + (size_t)g * N;
-    for (int k = threadIdx.x; k < N; k += blockDim.x) {
-        double s0 = 0.0, s1 = 0.0;
-        int i = 0;
-        for (; i + 2 <= N; i += 2) {
-            s0 += Q[(size_t)i * N + k] * rv[i];
-            s1 += Q[(size_t)(i + 1) * N + k] * rv[i + 1];
-        }
-        for (; i < N; ++i) s0 += Q[(size_t)i * N + k] * rv[i];
-        tmp[k] = (s0 + s1) / (fabs(lam[k] - theta) + a.gamma);
-    }
-    __syncthreads();
+    const double cl = 0.5 * (double)l * (double)(l + 1);
+    gs_colsum(a.T, x, N, y, scr);
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        double s0 = 0.0, s1 = 0.0;
-        int k = 0;
-        for (; k + 2 <= N; k += 2) {
-            s0 += QT[(size_t)k * N + i] * tmp[k];
-            s1 += QT[(size_t)(k + 1) * N + i] * tmp[k + 1];
-        }
-        for (; k < N; ++k) s0 += QT[(size_t)k * N + i] * tmp[k];
-        w[i] = s0 + s1;
+        const double ri = a.r[i];
+        const double d = cl / (ri * ri) - a.Z / ri + a.v[i];
+        y[i] = -0.5 * y[i] + d * x[i];
     }
+}
+
+// w = Q0 diag(1/(|lam0 - theta| + gamma)) Q0^T r, through tmp (all in LDS); the
+// eigenvector tables in FP32 (the preconditioner only sets the rate)
+static __device__ void gs_precondition(const GsArgs &a, int g, double theta, const double *rv,
+                                       double *tmp, double *w, double *scr) {
+    const int N = a.N;
+    const float *Q = a.Q0 + (size_t)g * N * N;    // [i][k]
+    const float *QT = a.Q0T + (size_t)g * N * N;  // [k][i]
+    const double *lam = a.lam0 + (size_t)g * N;
+    gs_colsum(Q, rv, N, tmp, scr);
+    for (int k = threadIdx.x; k < N; k += blockDim.x) tmp[k] /= fabs(lam[k] - theta) + a.gamma;
     __syncthreads();
+    gs_colsum(QT, tmp, N, w, scr);
 }
 
 static __device__ double gs_dot(const double *x, const double *y, int N, double *red) {
@@ -2760,11 +2772,13 @@ static __device__ void gs_deflate(const GsArgs &a, const int32_t *sh, int j, dou
 __global__ __launch_bounds__(GS_WG) void k_gs_sweep(GsArgs a) {
     extern __shared__ double gsm[];
     __shared__ double red[GS_WG / 64];
+    __shared__ double scr[GS_WG];
     const int g = blockIdx.x, N = a.N, l = a.gl[g];
     const int32_t *sh = a.gsh + (size_t)g * GS_MAX_K;
     int k = 0;
     while (k < GS_MAX_K && sh[k] >= 0) ++k;
     double *x = gsm, *hx = gsm + N, *w = gsm + 2 * N, *hw = gsm + 3 * N, *tmp = gsm + 4 * N;
+    double dmax = 0.0;
     for (int sw = 0; sw < a.sweeps; ++sw) {
         for (int j = 0; j < k; ++j) {
             double *us = a.u + (size_t)sh[j] * N;
@@ -2774,12 +2788,12 @@ __global__ __launch_bounds__(GS_WG) void k_gs_sweep(GsArgs a) {
             const double nx = sqrt(gs_dot(x, x, N, red));
             for (int i = threadIdx.x; i < N; i += blockDim.x) x[i] /= nx;
             __syncthreads();
-            gs_apply_h(a, l, x, hx);
+            gs_apply_h(a, l, x, hx, scr);
             __syncthreads();
             const double theta = gs_dot(x, hx, N, red);
             for (int i = threadIdx.x; i < N; i += blockDim.x) hw[i] = hx[i] - theta * x[i];  // residual
             __syncthreads();
-            gs_precondition(a, g, theta, hw, tmp, w);
+            gs_precondition(a, g, theta, hw, tmp, w, scr);
             gs_deflate(a, sh, j, w, red);
             const double cx = gs_dot(x, w, N, red);
             for (int i = threadIdx.x; i < N; i += blockDim.x) w[i] -= cx * x[i];
@@ -2790,7 +2804,7 @@ __global__ __launch_bounds__(GS_WG) void k_gs_sweep(GsArgs a) {
             if (nw > 1e-300) {
                 for (int i = threadIdx.x; i < N; i += blockDim.x) w[i] /= nw;
                 __syncthreads();
-                gs_apply_h(a, l, w, hw);
+                gs_apply_h(a, l, w, hw, scr);
                 __syncthreads();
                 const double b = gs_dot(x, hw, N, red);
                 const double c = gs_dot(w, hw, N, red);
@@ -2811,25 +2825,19 @@ __global__ __launch_bounds__(GS_WG) void k_gs_sweep(GsArgs a) {
             for (int i = threadIdx.x; i < N; i += blockDim.x) {
                 const double xn = ax * x[i] + aw * w[i];
                 x[i] = xn;
-                hx[i] = ax * hx[i] + aw * hw[i];
+                hx[i] = ax * hx[i] + aw * hw[i];  // H of the new vector, by linearity
                 us[i] = xn;
             }
             __syncthreads();
+            if (sw == a.sweeps - 1) {
+                // orbital energy with the final vector (later shells leave it
+                // unchanged) and the largest change (+inf at the first iteration)
+                const double e = gs_dot(x, hx, N, red);
+                dmax = fmax(dmax, fabs(e - a.eps[sh[j]]));
+                __syncthreads();
+                if (threadIdx.x == 0) a.eps[sh[j]] = e;
+            }
         }
-    }
-    // orbital energies with the final vectors, and the largest change
-    double dmax = 0.0;
-    for (int j = 0; j < k; ++j) {
-        const double *us = a.u + (size_t)sh[j] * N;
-        for (int i = threadIdx.x; i < N; i += blockDim.x) x[i] = us[i];
-        __syncthreads();
-        gs_apply_h(a, l, x, hx);
-        __syncthreads();
-        const double e = gs_dot(x, hx, N, red);
-        const double prev = a.eps[sh[j]];
-        dmax = fmax(dmax, fabs(e - prev));  // +inf at the first iteration
-        __syncthreads();
-        if (threadIdx.x == 0) a.eps[sh[j]] = e;
     }
     if (threadIdx.x == 0) a.res[1 + g] = dmax;
 }
@@ -2878,20 +2886,25 @@ extern "C" int ggb_ground_state(const ggb_gs_problem *p, const int32_t *shell_l,
     const size_t NN = (size_t)N * N;
     // transposed eigenvector tables and the start: u_s = the k-th bare-nucleus
     // eigenvector of its l (k = rank of s within its group)
-    std::vector<double> q0t((size_t)G * NN), u0((size_t)ns * N), einit(ns, INFINITY);
+    std::vector<float> q0f((size_t)G * NN), q0t((size_t)G * NN);
+    std::vector<double> u0((size_t)ns * N), einit(ns, INFINITY);
     for (int g = 0; g < G; ++g) {
         const double *Qg = Q0 + (size_t)g * NN;
-        double *QTg = q0t.data() + (size_t)g * NN;
+        float *QFg = q0f.data() + (size_t)g * NN, *QTg = q0t.data() + (size_t)g * NN;
         for (int i = 0; i < N; ++i)
-            for (int a = 0; a < N; ++a) QTg[(size_t)a * N + i] = Qg[(size_t)i * N + a];
+            for (int a = 0; a < N; ++a) {
+                QFg[(size_t)i * N + a] = (float)Qg[(size_t)i * N + a];
+                QTg[(size_t)a * N + i] = (float)Qg[(size_t)i * N + a];
+            }
         for (int k = 0; k < GS_MAX_K && gsh[(size_t)g * GS_MAX_K + k] >= 0; ++k) {
             const int s = gsh[(size_t)g * GS_MAX_K + k];
             for (int i = 0; i < N; ++i) u0[(size_t)s * N + i] = Qg[(size_t)i * N + k];
         }
     }
     hipStream_t st = nullptr;
-    double *dr = nullptr, *dwr = nullptr, *dT = nullptr, *dQ = nullptr, *dQT = nullptr, *dlam = nullptr,
-           *docc = nullptr, *du = nullptr, *deps = nullptr, *dv = nullptr, *drho = nullptr, *dres = nullptr;
+    double *dr = nullptr, *dwr = nullptr, *dT = nullptr, *dlam = nullptr, *docc = nullptr, *du = nullptr,
+           *deps = nullptr, *dv = nullptr, *drho = nullptr, *dres = nullptr;
+    float *dQ = nullptr, *dQT = nullptr;
     int32_t *dgsh = nullptr, *dgl = nullptr;
     int rc = GGB_OK;
     auto hip = [&](hipError_t e, const char *what) {
@@ -2913,8 +2926,8 @@ extern "C" int ggb_ground_state(const ggb_gs_problem *p, const int32_t *shell_l,
         hip(dalloc(&du, (size_t)ns * N), "alloc") && hip(dalloc(&deps, ns), "alloc") &&
         hip(dalloc(&dv, N), "alloc") && hip(dalloc(&drho, N), "alloc") && hip(dalloc(&dres, 1 + G), "alloc") &&
         hip(dalloc(&dgsh, (size_t)G * GS_MAX_K), "alloc") && hip(dalloc(&dgl, G), "alloc") &&
-        up(dr, r, N * 8) && up(dwr, wr, N * 8) && up(dT, T, NN * 8) && up(dQ, Q0, (size_t)G * NN * 8) &&
-        up(dQT, q0t.data(), (size_t)G * NN * 8) && up(dlam, lam0, (size_t)G * N * 8) &&
+        up(dr, r, N * 8) && up(dwr, wr, N * 8) && up(dT, T, NN * 8) && up(dQ, q0f.data(), (size_t)G * NN * 4) &&
+        up(dQT, q0t.data(), (size_t)G * NN * 4) && up(dlam, lam0, (size_t)G * N * 8) &&
         up(docc, occ, ns * 8) && up(du, u0.data(), (size_t)ns * N * 8) && up(deps, einit.data(), ns * 8) &&
         up(dgsh, gsh.data(), gsh.size() * 4) && up(dgl, gl.data(), G * 4)) {
         GsArgs a;
