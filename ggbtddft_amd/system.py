@@ -15,6 +15,7 @@ radial propagators, angular tables) with the north-star choices:
   L_H = 2 Lmax by default (exact multipoles of a density built from l <= Lmax).
 """
 import math
+import os
 
 import numpy as np
 
@@ -36,6 +37,15 @@ def manolopoulos_cap(r, r_mask, r_max):
     x = 2.0 * delta * kmin * d
     W[i0:] = emin * d * (a * x - b * x ** 3 + 4.0 / (c - x) ** 2 - 4.0 / (c + x) ** 2)
     return W
+
+
+def _blas_threads(n):
+    try:
+        from threadpoolctl import threadpool_limits
+        return threadpool_limits(limits=n)
+    except Exception:  # threadpoolctl absent: leave the BLAS as it is
+        import contextlib
+        return contextlib.nullcontext()
 
 
 def _round16(n):
@@ -69,11 +79,17 @@ class AtomSystem:
         self.interacting = self.gs.interacting
         self.v_gs = self.gs.v_gs
         self.ecut = ecut_factor * math.pi / self.dt
-        eig = []
-        for l in range(self.n_l):
+        def eig_l(l):
             lam, Q = np.linalg.eigh(self.grid.hamiltonian(l, self.atom.Z, self.v_gs))
             nk = int(np.searchsorted(lam, self.ecut, side="right"))
-            eig.append((lam[:nk], Q[:, :nk]))
+            return lam[:nk], Q[:, :nk]
+
+        # one dense eigendecomposition per l; LAPACK releases the GIL, so the
+        # l run on a thread pool, each with a single BLAS thread
+        from concurrent.futures import ThreadPoolExecutor
+        workers = max(1, min(16, os.cpu_count() or 1, self.n_l))
+        with _blas_threads(1), ThreadPoolExecutor(max_workers=workers) as ex:
+            eig = list(ex.map(eig_l, range(self.n_l)))
         self.nkeep = np.array([len(e[0]) for e in eig], dtype=np.int32)
         self.nk_max = _round16(max(1, self.nkeep.max()))
         self.Q = np.zeros((self.n_l, self.N, self.nk_max))
