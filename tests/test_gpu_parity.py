@@ -232,3 +232,32 @@ def test_kinetic_variants_agree_and_autotune(precision):
         name, times = prop.autotune(reps=2, cache=False)
         assert name in variants and set(times) == set(variants)
         assert np.array_equal(prop.download_orbitals(), before)
+
+
+@pytest.mark.parametrize("precision,n_inst", [(0, 1), (1, 12)])
+def test_timing_modes(precision, n_inst):
+    """Timing mode 1 stamps the two kinetic product kernels of every 16th step
+    (events from the kernels' own dispatch), mode 2 every phase of every step;
+    neither changes the results."""
+    _say("case: timing modes, Ne N=48 Lmax=6, precision %d, %d instance(s)" % (precision, n_inst))
+    dt = 0.05
+    S = AtomSystem("Ne", 48, 30.0, 6, dt)
+    t = (np.arange(32) + 0.5) * dt
+    E = np.stack([0.05 * np.sin(0.4 * t)] * n_inst) if n_inst > 1 else 0.05 * np.sin(0.4 * t)
+    eng = _engine()
+    outs = []
+    for mode in (0, 1, 2):
+        with eng.Propagator(S, n_inst=n_inst, precision=precision) as prop:
+            prop.set_field(E)
+            prop.set_timing(mode)
+            outs.append(prop.run(0, 32))
+            tim = prop.timing()
+        if mode == 1:
+            assert tim["kinetic_qt"][1] == 2 and tim["kinetic_q"][1] == 2
+            assert tim["kinetic_qt"][0] > 0.0 and tim["kinetic_q"][0] > 0.0
+            assert tim["potential_rebuild"][1] == 0
+        if mode == 2:
+            assert all(v[1] == 32 and v[0] > 0.0 for v in tim.values())
+    for o in outs[1:]:
+        for key in ("dip", "acc", "norm"):
+            assert np.array_equal(o[key], outs[0][key]), key
