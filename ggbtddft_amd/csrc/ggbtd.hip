@@ -48,6 +48,9 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 
 #define WG 256
 #define RT 8          // radial nodes per angular tile (16 real columns)
+#ifndef ANG_NT_MIXED
+#define ANG_NT_MIXED 2  // angular column tiles per work-group in the mixed form (1, 2 or 4)
+#endif
 #define RHO_MIN 1e-40 // v_xc = 0 at or below this density
 #define KIN_NARROW_MAX_C 8  // up to 8 complex columns: one 16-wide MFMA column tile
 #define TIMING_STRIDE 16    // timing mode 1 samples every 16th step
@@ -625,20 +628,23 @@ static __device__ __forceinline__ double c_lm(int l, int m) {
     return sqrt(((dl + 1.0) * (dl + 1.0) - dm * dm) / ((2.0 * dl + 1.0) * (2.0 * dl + 3.0)));
 }
 
-// cl[l] = c_lm(l, m) of the work-group's m, staged in LDS once
+// cl[l] = c_lm(l, m) of the work-group's m, staged in LDS once; X rows hold
+// XS = 16 NT doubles (RT NT radial nodes, re/im interleaved)
+template <int NT>
 static __device__ void ang_laser(double *X, int parity, int m, int Nl, const double *a_i,
                                  const double *cl) {
     // pairs (l, l+1), l = m + parity + 2 q, l + 1 < Nl; thread -> (q, i_local)
+    constexpr int R = RT * NT, XS = 16 * NT;
     const int t = threadIdx.x;
-    const int il = t & 7;
-    for (int q = t >> 3;; q += blockDim.x / 8) {
+    const int il = t % R;
+    for (int q = t / R;; q += blockDim.x / R) {
         const int l = m + parity + 2 * q;
         if (l + 1 >= Nl) break;
         const double th = a_i[il] * cl[l];
         double sn, cs;
         sincos(th, &sn, &cs);
-        double *x0 = X + l * 16 + 2 * il;
-        double *x1 = X + (l + 1) * 16 + 2 * il;
+        double *x0 = X + l * XS + 2 * il;
+        double *x1 = X + (l + 1) * XS + 2 * il;
         const double r0 = x0[0], i0 = x0[1], r1 = x1[0], i1 = x1[1];
         // [x0', x1'] = [[cs, -i sn], [-i sn, cs]] [x0, x1]
         x0[0] = cs * r0 + sn * i1;
@@ -660,12 +666,14 @@ struct ang_par {
 };
 
 // forward product of theta tile `tile` (rows j = 16 tile + 0..15 of the first
-// half): e = Bs X_s, o = Ba X_a; two independent accumulator chains.
-template <typename T>
+// half) for the NT column tiles: e[t] = Bs X_s, o[t] = Ba X_a; each table
+// value feeds NT products.
+template <typename T, int NT>
 static __device__ void ang_fwd(const T *Bs, const T *Ba, const double *X, const AngArgs &g,
-                               const ang_par &pr, int m, int tile, typename mfma_t<T>::v4 &e,
-                               typename mfma_t<T>::v4 &o) {
+                               const ang_par &pr, int m, int tile, typename mfma_t<T>::v4 (&e)[NT],
+                               typename mfma_t<T>::v4 (&o)[NT]) {
     typedef typename mfma_t<T>::v4 v4;
+    constexpr int XS = 16 * NT;
     const int lane = threadIdx.x & 63, kr = lane >> 4, lc = lane & 15;
     // rows j >= hq read row hq - 1: their results are never used (j >= h);
     // table columns k >= ns are zero and X rows up to Nl + 5 exist (zero), so
@@ -673,15 +681,23 @@ static __device__ void ang_fwd(const T *Bs, const T *Ba, const double *X, const 
     const int j = min(tile * 16 + lc, g.hq - 1);
     const T *As = Bs + (size_t)j * g.sS;
     const T *Aa = Ba + (size_t)j * g.sA;
-    v4 ae = (v4){0, 0, 0, 0}, ao = (v4){0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < NT; ++u) e[u] = (v4){0, 0, 0, 0}, o[u] = (v4){0, 0, 0, 0};
     const int kmax = pr.ks > pr.ka ? pr.ks : pr.ka;
     for (int k0 = 0; k0 < kmax; k0 += 4) {
         const int k = k0 + kr;
-        if (k0 < pr.ks) ae = mfma_t<T>::op(As[k], (T)X[(m + 2 * k) * 16 + lc], ae);
-        if (k0 < pr.ka) ao = mfma_t<T>::op(Aa[k], (T)X[(m + 1 + 2 * k) * 16 + lc], ao);
+        if (k0 < pr.ks) {
+            const T a = As[k];
+#pragma unroll
+            for (int u = 0; u < NT; ++u) e[u] = mfma_t<T>::op(a, (T)X[(m + 2 * k) * XS + 16 * u + lc], e[u]);
+        }
+        if (k0 < pr.ka) {
+            const T a = Aa[k];
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+                o[u] = mfma_t<T>::op(a, (T)X[(m + 1 + 2 * k) * XS + 16 * u + lc], o[u]);
+        }
     }
-    e = ae;
-    o = ao;
 }
 
 // row of X that output r of this lane's back-product item `item` (s tiles,
@@ -695,13 +711,14 @@ static __device__ int ang_bwd_row(const ang_par &pr, int m, int item, int r) {
     return ko < (sym ? pr.ns : pr.na) ? m + (sym ? 0 : 1) + 2 * ko : -1;
 }
 
-// back product of output tile `item` (s tiles, then a tiles): v[r] for the
-// rows ang_bwd_row gives; two accumulator chains over alternating groups of
-// 4 nodes, added at the end.
-template <typename T>
+// back product of output tile `item` (s tiles, then a tiles) for the NT column
+// tiles: v[t][r] for the rows ang_bwd_row gives; two accumulator chains per
+// column tile over alternating groups of 4 nodes, added at the end.
+template <typename T, int NT>
 static __device__ void ang_bwd_sum(const T *Bs, const T *Ba, const T *S, const T *D,
-                                   const AngArgs &g, const ang_par &pr, int item, double v[4]) {
+                                   const AngArgs &g, const ang_par &pr, int item, double (&v)[NT][4]) {
     typedef typename mfma_t<T>::v4 v4;
+    constexpr int XS = 16 * NT;
     const int lane = threadIdx.x & 63, kr = lane >> 4, lc = lane & 15;
     const int nst = (pr.ns + 15) / 16;
     const bool sym = item < nst;
@@ -712,43 +729,37 @@ static __device__ void ang_bwd_sum(const T *Bs, const T *Ba, const T *S, const T
     const int kk = sym ? pr.ks : pr.ka;
     // output rows k >= kk read column kk - 1 (in range); they are not stored
     const T *Ac = A + min(tile * 16 + lc, kk - 1);
-    v4 a0 = (v4){0, 0, 0, 0}, a1 = (v4){0, 0, 0, 0};
+    v4 a0[NT], a1[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) a0[u] = (v4){0, 0, 0, 0}, a1[u] = (v4){0, 0, 0, 0};
     int j0 = 0;
     for (; j0 + 8 <= g.hq; j0 += 8) {
         const int ja = j0 + kr, jb = j0 + 4 + kr;
-        a0 = mfma_t<T>::op(Ac[(size_t)ja * st], Pv[ja * 16 + lc], a0);
-        a1 = mfma_t<T>::op(Ac[(size_t)jb * st], Pv[jb * 16 + lc], a1);
+        const T xa = Ac[(size_t)ja * st], xb = Ac[(size_t)jb * st];
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            a0[u] = mfma_t<T>::op(xa, Pv[ja * XS + 16 * u + lc], a0[u]);
+            a1[u] = mfma_t<T>::op(xb, Pv[jb * XS + 16 * u + lc], a1[u]);
+        }
     }
     if (j0 < g.hq) {
         const int ja = j0 + kr;
-        a0 = mfma_t<T>::op(Ac[(size_t)ja * st], Pv[ja * 16 + lc], a0);
+        const T xa = Ac[(size_t)ja * st];
+#pragma unroll
+        for (int u = 0; u < NT; ++u) a0[u] = mfma_t<T>::op(xa, Pv[ja * XS + 16 * u + lc], a0[u]);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = (double)(a0[r] + a1[r]);
-}
-
-// back product into X (mixed form: added to the values there)
-template <typename T>
-static __device__ void ang_bwd(const T *Bs, const T *Ba, const T *S, const T *D, double *X,
-                               const AngArgs &g, const ang_par &pr, int m, int item) {
-    double v[4];
-    ang_bwd_sum<T>(Bs, Ba, S, D, g, pr, item, v);
-    const int lc = threadIdx.x & 15;
+    for (int u = 0; u < NT; ++u)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int l = ang_bwd_row<T>(pr, m, item, r);
-        if (l >= 0) {
-            double &x = X[l * 16 + lc];
-            x = mfma_t<T>::mixed ? x + v[r] : v[r];
-        }
-    }
+        for (int r = 0; r < 4; ++r) v[u][r] = (double)(a0[u][r] + a1[u][r]);
 }
 
-template <int MODE, typename T, bool GLB>
+template <int MODE, typename T, bool GLB, int NT>
 __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     typedef typename mfma_t<T>::v4 v4;
+    constexpr int R = RT * NT, XS = 16 * NT;  // radial nodes and X row length of the work-group
     extern __shared__ __align__(16) double lds[];
-    __shared__ double a_i[RT];
+    __shared__ double a_i[R];
     __shared__ double cl[ANG_MAX_NL];
     __shared__ double red[3][ANG_MAXW];
 
@@ -758,28 +769,30 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     const int korb = lg % g.n_orb, rest = lg / g.n_orb;
     const int tile_i = rest % gridDim.x, inst = rest / gridDim.x;
     const int c = inst * g.n_orb + korb;
-    const int i0 = tile_i * RT;
+    const int i0 = tile_i * R;
     const int m = g.mcol[c % g.n_orb];
     const int t = threadIdx.x, nthr = blockDim.x;
     const int wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
-    const int ilc = lc >> 1, p = lc & 1;  // column -> (i_local, re/im)
+    const int ilc = lc >> 1, p = lc & 1;  // column -> (i_local within the column tile, re/im)
     const ang_par pr(g.Nl, m);
     const int mt = (g.h + 15) / 16;
     const int nbt = (pr.ns + 15) / 16 + (pr.na + 15) / 16;
 
     const T *Bsrc = reinterpret_cast<const T *>(g.Bh) + (size_t)m * g.nblk;
     T *Tl = reinterpret_cast<T *>(lds);
-    double *X = reinterpret_cast<double *>(GLB ? Tl : Tl + g.nblk);  // [Nlx][16]
+    double *X = reinterpret_cast<double *>(GLB ? Tl : Tl + g.nblk);  // [Nlx][XS]
     const bool AL = mfma_t<T>::mixed && g.alias;                     // S/D inside X
-    T *S = reinterpret_cast<T *>(AL ? X : X + g.Nlx * 16);           // [hq][16]
-    T *D = S + g.hq * 16;                                            // [hq][16]
+    T *S = reinterpret_cast<T *>(AL ? X : X + g.Nlx * XS);           // [hq][XS]
+    T *D = S + g.hq * XS;                                            // [hq][XS]
     const T *Bs = GLB ? Bsrc : Tl;
     const T *Ba = Bs + (size_t)g.hq * g.sS;
 
-    const int i = i0 + ilc;
-    const bool iok = i < g.N;
-    const size_t pbase = ((size_t)inst * g.N + (iok ? i : 0)) * g.Nth;
+    // radial node of this lane in column tile u: i0 + 8 u + ilc
+    int iu[NT];
+    bool oku[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) iu[u] = i0 + RT * u + ilc, oku[u] = iu[u] < g.N;
     if (!GLB) {
         // the block of one m is a multiple of 16 bytes (host pads nblk)
         const int n16 = g.nblk * (int)sizeof(T) / 16;
@@ -788,15 +801,15 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         for (int idx = t; idx < n16; idx += nthr) d4[idx] = s4[idx];
     }
     // load the l-space tile
-    for (int idx = t; idx < g.Nlx * RT; idx += nthr) {
-        const int l = idx / RT, il = idx % RT;
+    for (int idx = t; idx < g.Nlx * R; idx += nthr) {
+        const int l = idx / R, il = idx % R;
         const int ii = i0 + il;
         double2 v = make_double2(0.0, 0.0);
         if (l < g.Nl && ii < g.N) v = ((const double2 *)g.psi)[((size_t)l * g.C + c) * g.N + ii];
-        X[l * 16 + 2 * il] = v.x;
-        X[l * 16 + 2 * il + 1] = v.y;
+        X[l * XS + 2 * il] = v.x;
+        X[l * XS + 2 * il + 1] = v.y;
     }
-    if (MODE != 2 && t < RT) {
+    if (MODE != 2 && t < R) {
         const int i = i0 + t;
         const int fs = g.cur ? (int)(((long)g.cur[0] + g.step) % g.n_field) : g.step;
         const double Ev = g.E[(size_t)inst * g.n_field + fs];
@@ -807,9 +820,9 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     __syncthreads();
 
     if (MODE == 1) {
-        ang_laser(X, 1, m, g.Nl, a_i, cl);
+        ang_laser<NT>(X, 1, m, g.Nl, a_i, cl);
         __syncthreads();
-        ang_laser(X, 0, m, g.Nl, a_i, cl);
+        ang_laser<NT>(X, 0, m, g.Nl, a_i, cl);
         __syncthreads();
     }
 
@@ -819,52 +832,57 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         // old tile values are taken first, S/D written after a barrier, the
         // back products formed in registers, and the tile rewritten after a
         // barrier (rows l < m and l >= N_l back to zero)
-        T svr[4], dvr[4];
+        T svr[NT][4], dvr[NT][4];
         if (wave < mt) {
-            v4 e, o;
-            ang_fwd<T>(Bs, Ba, X, g, pr, m, wave, e, o);
+            v4 e[NT], o[NT];
+            ang_fwd<T, NT>(Bs, Ba, X, g, pr, m, wave, e, o);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int j = wave * 16 + mfma_t<T>::row(kr, r);
-                const int jm = g.Nth - 1 - j;
-                const T pa = e[r] + o[r], pb = e[r] - o[r];
-                const T pao = __shfl_xor(pa, 1), pbo = __shfl_xor(pb, 1);
-                T sv = (T)0, dv = (T)0;
-                if (j < g.h && iok) {
-                    T fax, fay, fbx, fby;
-                    if (mfma_t<T>::mixed) {
-                        // mixed form: the table holds (f - 1), formed in FP64 before the rounding
-                        const float2 *ph = reinterpret_cast<const float2 *>(g.phase) + pbase;
-                        const float2 fa_ = ph[j], fb_ = ph[jm];
-                        fax = fa_.x, fay = fa_.y, fbx = fb_.x, fby = fb_.y;
-                    } else {
-                        const double2 *ph = reinterpret_cast<const double2 *>(g.phase) + pbase;
-                        const double2 fa_ = ph[j], fb_ = ph[jm];
-                        fax = (T)fa_.x, fay = (T)fa_.y, fbx = (T)fb_.x, fby = (T)fb_.y;
+            for (int u = 0; u < NT; ++u) {
+                const size_t pbase = ((size_t)inst * g.N + (oku[u] ? iu[u] : 0)) * g.Nth;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = wave * 16 + mfma_t<T>::row(kr, r);
+                    const int jm = g.Nth - 1 - j;
+                    const T pa = e[u][r] + o[u][r], pb = e[u][r] - o[u][r];
+                    const T pao = __shfl_xor(pa, 1), pbo = __shfl_xor(pb, 1);
+                    T sv = (T)0, dv = (T)0;
+                    if (j < g.h && oku[u]) {
+                        T fax, fay, fbx, fby;
+                        if (mfma_t<T>::mixed) {
+                            // mixed form: the table holds (f - 1), formed in FP64 before the rounding
+                            const float2 *ph = reinterpret_cast<const float2 *>(g.phase) + pbase;
+                            const float2 fa_ = ph[j], fb_ = ph[jm];
+                            fax = fa_.x, fay = fa_.y, fbx = fb_.x, fby = fb_.y;
+                        } else {
+                            const double2 *ph = reinterpret_cast<const double2 *>(g.phase) + pbase;
+                            const double2 fa_ = ph[j], fb_ = ph[jm];
+                            fax = (T)fa_.x, fay = (T)fa_.y, fbx = (T)fb_.x, fby = (T)fb_.y;
+                        }
+                        // component p of f (x + i y): x fx - y fy (p = 0), y fx + x fy (p = 1);
+                        // this lane holds component p, its partner the other
+                        const T sg = p ? (T)1 : (T)-1;
+                        const T qa = pa * fax + sg * pao * fay;
+                        const T qb = pb * fbx + sg * pbo * fby;
+                        sv = (jm == j) ? qa : qa + qb;
+                        dv = (jm == j) ? qa : qa - qb;
                     }
-                    // component p of f (x + i y): x fx - y fy (p = 0), y fx + x fy (p = 1);
-                    // this lane holds component p, its partner the other
-                    const T sg = p ? (T)1 : (T)-1;
-                    const T qa = pa * fax + sg * pao * fay;
-                    const T qb = pb * fbx + sg * pbo * fby;
-                    sv = (jm == j) ? qa : qa + qb;
-                    dv = (jm == j) ? qa : qa - qb;
-                }
-                svr[r] = sv;
-                dvr[r] = dv;
-                if (!AL && j < g.hq) {
-                    S[j * 16 + lc] = sv;
-                    D[j * 16 + lc] = dv;
+                    svr[u][r] = sv;
+                    dvr[u][r] = dv;
+                    if (!AL && j < g.hq) {
+                        S[j * XS + 16 * u + lc] = sv;
+                        D[j * XS + 16 * u + lc] = dv;
+                    }
                 }
             }
         }
+        double xo[NT][4];
         if (AL) {
-            double xo[4];
             if (wave < nbt) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int l = ang_bwd_row<T>(pr, m, wave, r);
-                    xo[r] = l >= 0 ? X[l * 16 + lc] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < NT; ++u) xo[u][r] = l >= 0 ? X[l * XS + 16 * u + lc] : 0.0;
                 }
             }
             __syncthreads();
@@ -873,49 +891,58 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                 for (int r = 0; r < 4; ++r) {
                     const int j = wave * 16 + mfma_t<T>::row(kr, r);
                     if (j < g.hq) {
-                        S[j * 16 + lc] = svr[r];
-                        D[j * 16 + lc] = dvr[r];
+#pragma unroll
+                        for (int u = 0; u < NT; ++u) {
+                            S[j * XS + 16 * u + lc] = svr[u][r];
+                            D[j * XS + 16 * u + lc] = dvr[u][r];
+                        }
                     }
                 }
             }
-            __syncthreads();
-            double v[4];
-            if (wave < nbt) ang_bwd_sum<T>(Bs, Ba, S, D, g, pr, wave, v);
+        }
+        __syncthreads();
+        double v[NT][4];
+        if (wave < nbt) ang_bwd_sum<T, NT>(Bs, Ba, S, D, g, pr, wave, v);
+        if (AL) {
             __syncthreads();
             const int npad = g.Nlx - g.Nl;
-            for (int idx = t; idx < (m + npad) * 16; idx += nthr) {
-                const int row = idx / 16 < m ? idx / 16 : g.Nl + idx / 16 - m;
-                X[row * 16 + idx % 16] = 0.0;
+            for (int idx = t; idx < (m + npad) * XS; idx += nthr) {
+                const int row = idx / XS < m ? idx / XS : g.Nl + idx / XS - m;
+                X[row * XS + idx % XS] = 0.0;
             }
-            if (wave < nbt) {
+        }
+        if (wave < nbt) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int l = ang_bwd_row<T>(pr, m, wave, r);
-                    if (l >= 0) X[l * 16 + lc] = xo[r] + v[r];
+            for (int r = 0; r < 4; ++r) {
+                const int l = ang_bwd_row<T>(pr, m, wave, r);
+                if (l >= 0) {
+#pragma unroll
+                    for (int u = 0; u < NT; ++u) {
+                        double &x = X[l * XS + 16 * u + lc];
+                        // mixed form: added to the old values (in registers when S/D overlay X)
+                        x = !mfma_t<T>::mixed ? v[u][r] : (AL ? xo[u][r] : x) + v[u][r];
+                    }
                 }
             }
-        } else {
-            __syncthreads();
-            if (wave < nbt) ang_bwd<T>(Bs, Ba, S, D, X, g, pr, m, wave);
         }
         __syncthreads();
     }
 
     if (MODE == 0) {
-        ang_laser(X, 0, m, g.Nl, a_i, cl);
+        ang_laser<NT>(X, 0, m, g.Nl, a_i, cl);
         __syncthreads();
-        ang_laser(X, 1, m, g.Nl, a_i, cl);
+        ang_laser<NT>(X, 1, m, g.Nl, a_i, cl);
         __syncthreads();
     }
 
     if (MODE != 2) {
         const int lref = (MODE == 0 && g.psi32 && g.l0) ? g.l0[korb] : -1;
-        for (int idx = t; idx < g.Nl * RT; idx += nthr) {
-            const int l = idx / RT, il = idx % RT;
+        for (int idx = t; idx < g.Nl * R; idx += nthr) {
+            const int l = idx / R, il = idx % R;
             const int ii = i0 + il;
             if (ii < g.N) {
                 const size_t o = ((size_t)l * g.C + c) * g.N + ii;
-                const double xr = X[l * 16 + 2 * il], xi = X[l * 16 + 2 * il + 1];
+                const double xr = X[l * XS + 2 * il], xi = X[l * XS + 2 * il + 1];
                 ((double2 *)g.psi)[o] = make_double2(xr, xi);
                 if (MODE == 0 && g.psi32 && l != lref) g.psi32[o] = make_float2((float)xr, (float)xi);
             }
@@ -924,32 +951,35 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     if (MODE == 0) return;
 
     // |B psi|^2 and observable partials (accumulated in FP64)
-    const double ri = iok ? g.r[i] : 1.0;
-    const double ir2 = 1.0 / (ri * ri);
     double sd = 0.0, sa = 0.0, sn = 0.0;
     if (wave < mt) {
-        v4 e, o;
-        ang_fwd<T>(Bs, Ba, X, g, pr, m, wave, e, o);
-        T *dens = reinterpret_cast<T *>(g.dens) + ((size_t)c * g.N + (iok ? i : 0)) * g.Nth;
+        v4 e[NT], o[NT];
+        ang_fwd<T, NT>(Bs, Ba, X, g, pr, m, wave, e, o);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int j = wave * 16 + mfma_t<T>::row(kr, r);
-            const int jm = g.Nth - 1 - j;
-            const double ed = (double)e[r], od = (double)o[r];
-            const double va = ed + od, vb = ed - od;
-            const double s2a = va * va + __shfl_xor(va * va, 1);
-            const double s2b = vb * vb + __shfl_xor(vb * vb, 1);
-            const double eo = 4.0 * (ed * od + __shfl_xor(ed * od, 1));  // |Psi_j|^2 - |Psi_j'|^2
-            if (p == 0 && iok && j < g.h) {
-                dens[j] = (T)s2a;
-                if (jm != j) {
-                    const double ca = g.chi[j];
-                    dens[jm] = (T)s2b;
-                    sd += ri * ca * eo;
-                    sa += ca * eo * ir2;
-                    sn += s2a + s2b;
-                } else {
-                    sn += s2a;  // middle node, chi = 0
+        for (int u = 0; u < NT; ++u) {
+            const double ri = oku[u] ? g.r[iu[u]] : 1.0;
+            const double ir2 = 1.0 / (ri * ri);
+            T *dens = reinterpret_cast<T *>(g.dens) + ((size_t)c * g.N + (oku[u] ? iu[u] : 0)) * g.Nth;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = wave * 16 + mfma_t<T>::row(kr, r);
+                const int jm = g.Nth - 1 - j;
+                const double ed = (double)e[u][r], od = (double)o[u][r];
+                const double va = ed + od, vb = ed - od;
+                const double s2a = va * va + __shfl_xor(va * va, 1);
+                const double s2b = vb * vb + __shfl_xor(vb * vb, 1);
+                const double eo = 4.0 * (ed * od + __shfl_xor(ed * od, 1));  // |Psi_j|^2 - |Psi_j'|^2
+                if (p == 0 && oku[u] && j < g.h) {
+                    dens[j] = (T)s2a;
+                    if (jm != j) {
+                        const double ca = g.chi[j];
+                        dens[jm] = (T)s2b;
+                        sd += ri * ca * eo;
+                        sa += ca * eo * ir2;
+                        sn += s2a + s2b;
+                    } else {
+                        sn += s2a;  // middle node, chi = 0
+                    }
                 }
             }
         }
@@ -1428,6 +1458,8 @@ struct ang_geom {
     size_t lds;  // dynamic LDS bytes
     bool glb;    // tables read from global memory
     bool alias;  // mixed form: S/D share the l-space tile's LDS
+    int nt;      // column tiles (of RT radial nodes) per work-group
+    int ntiles;  // work-groups along r
 };
 static ang_geom ang_geometry(const ggb_problem *p) {
     ang_geom a;
@@ -1445,11 +1477,17 @@ static ang_geom ang_geometry(const ggb_problem *p) {
     // mixed form: the folded sums S/D (FP32) fit in the FP64 l-space tile's
     // space whenever hq <= N_l + 6; the kernel parks the old tile values it
     // needs in registers meanwhile (one work-group less of LDS per CU)
-    const size_t xbytes = (size_t)(p->n_l + 6) * 16 * sizeof(double);
-    const size_t sdbytes = (size_t)2 * a.hq * 16 * es;
+    // mixed form (many columns): two column tiles per work-group, each table
+    // value then feeds two products and the per-work-group work is shared by
+    // twice the radial nodes
+    a.nt = p->precision ? ANG_NT_MIXED : 1;
+    static_assert(ANG_NT_MIXED == 1 || ANG_NT_MIXED == 2 || ANG_NT_MIXED == 4, "column tiles: 1, 2 or 4");
+    a.ntiles = (p->n_r + RT * a.nt - 1) / (RT * a.nt);
+    const size_t xbytes = (size_t)(p->n_l + 6) * 16 * a.nt * sizeof(double);
+    const size_t sdbytes = (size_t)2 * a.hq * 16 * a.nt * es;
     a.alias = p->precision && sdbytes <= xbytes;
     const size_t tiles = a.alias ? xbytes : xbytes + sdbytes;
-    const size_t stat = (RT + ANG_MAX_NL + 3 * ANG_MAXW) * sizeof(double);
+    const size_t stat = (RT * a.nt + ANG_MAX_NL + 3 * ANG_MAXW) * sizeof(double);
 #ifndef ANG_FORCE_GLB
 #define ANG_FORCE_GLB 0
 #endif
@@ -1868,12 +1906,33 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
 
 template <int MODE, typename T>
 static void launch_angular_t(ggb_ctx *ctx, const AngArgs &g) {
-    const dim3 ga(ctx->ntiles, ctx->C);
     const ang_geom &ag = ctx->ang;
-    if (ag.glb)
-        k_angular<MODE, T, true><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
-    else
-        k_angular<MODE, T, false><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
+    const dim3 ga(ag.ntiles, ctx->C);
+    // the geometry (ag.nt, ag.ntiles, ag.lds) was computed for this tile count
+    switch (ag.nt) {
+        case 1:
+            if (ag.glb)
+                k_angular<MODE, T, true, 1><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
+            else
+                k_angular<MODE, T, false, 1><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
+            break;
+        case 2:
+            if (ag.glb)
+                k_angular<MODE, T, true, 2><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
+            else
+                k_angular<MODE, T, false, 2><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
+            break;
+#if ANG_NT_MIXED == 4
+        case 4:
+            if (ag.glb)
+                k_angular<MODE, T, true, 4><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
+            else
+                k_angular<MODE, T, false, 4><<<ga, 64 * ag.waves, ag.lds, ctx->stream>>>(g);
+            break;
+#endif
+        default:
+            break;  // ggb_create accepts only the tile counts above
+    }
 }
 
 static int kin_variant(const ggb_ctx *ctx);
@@ -2059,7 +2118,7 @@ static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int
     ob.part = ctx->obs_part;
     ob.obs = obs_dst;
     ob.C = ctx->C;
-    ob.ntiles = ctx->ntiles;
+    ob.ntiles = ctx->ang.ntiles;  // observable partials per column: one per angular work-group along r
     ob.cur = cur;
     ob.origin = cur ? cur + 1 : nullptr;
     ob.local = step;
