@@ -13,7 +13,11 @@ tables resident in HBM; the K steps are bracketed by stream synchronisation.
 The dominant kernel (the kinetic propagator pass) is timed with HIP events on
 its stream inside the timed region for the roofline entry.  The CPU baseline
 is the FP64 oracle on the host cores for a bounded window of the same
-workload.
+workload.  Further entries of the JSON line: the batched FP32 leg (64
+instances, ratio to the FP64 value), the serving leg (p50/p99 time to
+spectrum), the device spectrum and ground state against their host
+counterparts, and the precision report (full-length FP32 runs, one instance
+and batched, against FP64: harmonic-intensity errors).
 """
 import argparse
 import json
@@ -279,8 +283,9 @@ def main():
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-slots", type=int, default=8)
-    ap.add_argument("--precision-report", action="store_true",
-                    help="full-length FP64 and FP32 runs; harmonic-intensity errors")
+    ap.add_argument("--no-precision-report", action="store_true",
+                    help="skip the full-length FP64 / FP32 / batched-FP32 runs and their "
+                         "harmonic-intensity errors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (profiler byte-counter "
                          "calibration) and print its bandwidth")
@@ -384,7 +389,7 @@ def main():
         line["serving"] = serving_run(sysm, args)
     line["spectrum"] = spectrum_timing(nfull)
     line["ground_state"] = gs_stats
-    if args.precision_report:
+    if not args.no_precision_report:
         line["precision_report"] = precision_report(sysm, pulse)
     if not args.no_cpu_baseline:
         try:
