@@ -451,6 +451,13 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
     load(0);
     stash(0);
     __syncthreads();
+    // two-level sum: each chunk's products in acc, added to hi after the chunk,
+    // so no FP32 accumulation chain is longer than one chunk plus the chunk count
+    v4 hi[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) hi[x][y] = (v4){0, 0, 0, 0};
     int buf = 0;
     for (int k0 = 0; k0 < N; k0 += TKT) {
         const bool more = k0 + TKT < N;
@@ -458,6 +465,10 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
 #pragma unroll
         for (int ks = 0; ks < TKT / 4; ++ks) kin_tile_step<T>(As[buf], Bs[buf], ks * 4 + kr, wm, wn, lc, acc);
         if (more) stash(buf ^ 1);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) hi[x][y] += acc[x][y], acc[x][y] = (v4){0, 0, 0, 0};
         __syncthreads();
         buf ^= 1;
     }
@@ -473,7 +484,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int a = ab + mfma_t<T>::row(kr, r);
-                const double v = (double)acc[x][y][r];
+                const double v = (double)hi[x][y][r];
                 const double vo = __shfl_xor(v, 1);
                 o[r] = kin_factor<T>(v, vo, p, kphl[a < nk ? a : 0]);
             }
@@ -540,6 +551,13 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
     load(0);
     stash(0);
     __syncthreads();
+    // two-level sum: each chunk's products in acc, added to hi after the chunk,
+    // so no FP32 accumulation chain is longer than one chunk plus the chunk count
+    v4 hi[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) hi[x][y] = (v4){0, 0, 0, 0};
     int buf = 0;
     for (int k0 = 0; k0 < nk; k0 += TKT) {
         const bool more = k0 + TKT < nk;
@@ -547,6 +565,10 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
 #pragma unroll
         for (int ks = 0; ks < TKT / 4; ++ks) kin_tile_step<T>(As[buf], Bs[buf], ks * 4 + kr, wm, wn, lc, acc);
         if (more) stash(buf ^ 1);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) hi[x][y] += acc[x][y], acc[x][y] = (v4){0, 0, 0, 0};
         __syncthreads();
         buf ^= 1;
     }
@@ -564,7 +586,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
             for (int r = 0; r < 4; ++r) {
                 const int i = i0 + wm + 16 * x + mfma_t<T>::row(kr, r);
                 if (i < N) {
-                    double v = (double)acc[x][y][r];
+                    double v = (double)hi[x][y][r];
                     if (rrow) v += rc * rrow[i];
                     Xl[((size_t)c * N + i) * 2 + p] = v;
                 }
