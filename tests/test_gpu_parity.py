@@ -92,9 +92,12 @@ def test_substeps(atom, N, R, lmax):
 
 
 @pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30),
-                                                   ("Kr", 64, 40.0, 8, 20), ("Xe", 64, 40.0, 10, 12)])
+                                                   ("Kr", 64, 40.0, 8, 20), ("Xe", 64, 40.0, 10, 12),
+                                                   ("He", 32, 30.0, 96, 10)])
 def test_short_run(atom, N, R, lmax, nsteps):
-    # Kr: 13 orbitals with |m| up to 2; Xe: 4d shells, three |m| groups
+    # Kr: 13 orbitals with |m| up to 2; Xe: 4d shells, three |m| groups; He at
+    # Lmax = 96: the FP64 angular tables exceed the LDS budget and are read
+    # from global memory
     _say("case: short run", atom, N, lmax, nsteps)
     dt = 0.05
     S, O = _pair(atom, N, R, lmax, dt)
@@ -152,7 +155,8 @@ def test_batched_instances_match_alone(n_inst):
         assert np.abs(psi_mix[:, cols, :] - psi_alone).max() < 1e-12
 
 
-@pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30)])
+@pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30),
+                                                   ("He", 32, 30.0, 96, 10)])
 def test_fp32_short_run(atom, N, R, lmax, nsteps):
     """Mixed precision (FP32 tables/transforms/v_xc, FP64 state and sums)
     against the FP64 oracle: FP32-level agreement."""
