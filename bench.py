@@ -128,12 +128,18 @@ def serving_run(sysm_ar, args):
     reqs = []
     for k in range(n):
         I = float(np.exp(rng.uniform(np.log(5e13), np.log(5e14))))
-        lam = float(rng.uniform(600.0, 1000.0))
+        lam = float(rng.uniform(800.0, 2000.0))
+        phase = float(rng.uniform(0.0, 2.0 * np.pi))
         atom, nr, rmax, lmax, dt, two = kinds[k % len(kinds)]
+        if two:
+            # in-situ two-colour request: 800 nm at 1.5e14 W/cm^2 with its second
+            # harmonic at 1e-3 relative intensity and a random relative phase
+            I, lam, second = 1.5e14, 800.0, (2, 1e-3, phase)
+        else:
+            second = None
         reqs.append(Request(rid=k, atom=atom, intensity=I, wavelength=lam, n_cycles=2, N=nr,
-                            r_max=rmax, lmax=lmax, dt=dt, arrival=float(arr[k]),
-                            second=(2, 0.1, 0.0) if two else None))
-    srv = Server(slots=args.serving_slots, chunk=256, ring=1 << 15, precision=1)
+                            r_max=rmax, lmax=lmax, dt=dt, arrival=float(arr[k]), second=second))
+    srv = Server(slots=args.serving_slots, chunk=256, ring=1 << 16, precision=1)
     srv.systems[(ATOM, N, float(R_MAX), LMAX, float(DT))] = sysm_ar
     t0 = time.perf_counter()
     for key in {r.key() for r in reqs}:   # ground states and tables before the clock starts
@@ -146,8 +152,10 @@ def serving_run(sysm_ar, args):
     stats.update({"requests": n, "arrival_rate_per_s": args.serving_rate,
                   "slots_per_lane": args.serving_slots, "precision": "fp32 mixed",
                   "workload": "Poisson stream (seed 0): He N=256 Lmax=32, Ne N=384 Lmax=48 "
-                              "(dt=0.05), two-colour Ar N=512 Lmax=64 and Kr N=768 Lmax=96 "
-                              "(dt=0.02); 2-cycle pulses 5e13-5e14 W/cm^2, 600-1000 nm",
+                              "(dt=0.05) and Kr N=768 Lmax=96 (dt=0.02), one colour "
+                              "800-2000 nm, 5e13-5e14 W/cm^2; Ar N=512 Lmax=64 (dt=0.02) "
+                              "in situ: 800 nm at 1.5e14 W/cm^2 + 400 nm at 1e-3 relative "
+                              "intensity, random relative phase; 2-cycle pulses",
                   "system_setup_s_before_clock": t_sys,
                   "bad_results": len(bad)})
     log("serving: %d requests, p50 %.2f s, p99 %.2f s" % (n, stats["p50_time_to_spectrum_s"],
