@@ -95,6 +95,8 @@ class Lane:
         if hasattr(self.prop, "autotune"):  # kinetic tile shapes for this lane's shape
             self.prop.autotune(reps=3)
         self.prop.field_ring(ring)
+        if hasattr(self.prop, "set_graph"):  # each chunk replays a captured run of steps
+            self.prop.set_graph(min(64, chunk))
         self.step = 0
         self.active = [None] * slots
         self.orbital_steps = 0
