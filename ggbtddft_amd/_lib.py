@@ -18,7 +18,7 @@ ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "call ou
 EXPORTS = ("ggb_abi_version", "ggb_create", "ggb_upload_static", "ggb_upload_orbitals",
            "ggb_set_field", "ggb_run", "ggb_download_orbitals", "ggb_download_potential",
            "ggb_debug_substep", "ggb_set_timing", "ggb_get_timing", "ggb_selftest_mfma",
-           "ggb_field_ring", "ggb_set_field_slice", "ggb_upload_columns",
+           "ggb_field_ring", "ggb_set_field_slice", "ggb_upload_columns", "ggb_set_active",
            "ggb_rebuild_potential", "ggb_footprint", "ggb_selftest_copy", "ggb_set_graph",
            "ggb_spectrum", "ggb_set_kinetic_variant", "ggb_autotune", "ggb_ground_state",
            "ggb_last_error", "ggb_destroy")
@@ -101,6 +101,8 @@ def load():
     lib.ggb_set_field_slice.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp]
     lib.ggb_upload_columns.restype = ctypes.c_int
     lib.ggb_upload_columns.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _dp]
+    lib.ggb_set_active.restype = ctypes.c_int
+    lib.ggb_set_active.argtypes = [vp, _ip]
     lib.ggb_rebuild_potential.restype = ctypes.c_int
     lib.ggb_rebuild_potential.argtypes = [vp]
     lib.ggb_footprint.restype = ctypes.c_int

@@ -125,7 +125,19 @@ struct KinRef {
     const double *phi;   // [n_orb][N]
     const int32_t *l0;   // [n_orb]
     const double4 *ab;   // [C]
+    const int32_t *active;  // [n_inst] instances in use, or null (all)
 };
+
+// whether any instance with a column among real columns [n0, n0 + nreal) is
+// in use (work-group-uniform; idle serving slots skip their kinetic tiles)
+static __device__ __forceinline__ bool kin_cols_active(const KinRef &kr, int n0, int nreal, int C,
+                                                       int n_orb) {
+    if (!kr.active) return true;
+    const int c0 = n0 >> 1, c1 = min(C, (n0 + nreal + 1) >> 1);
+    for (int q = c0 / n_orb; q * n_orb < c1; ++q)
+        if (kr.active[q]) return true;
+    return false;
+}
 __global__ __launch_bounds__(WG) void k_kin_ref(const double *__restrict__ psi,
                                                 const double *__restrict__ phi,
                                                 const int32_t *__restrict__ l0,
@@ -222,6 +234,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
     const int nk = nkeep[l];
     const int a0 = tb.x * 16;
     if (a0 >= nk) return;  // block-uniform
+    if (!kin_cols_active(ref, tb.y * 16, 16, C, n_orb)) return;  // idle instances only
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int col = tb.y * 16 + lc;
@@ -300,6 +313,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ QT,
     const int l = tb.z;
     const int nk = nkeep[l];
     const int kend = min(nk_max, (nk + 15) & ~15);
+    if (!kin_cols_active(ref, tb.y * 16, 16, C, n_orb)) return;  // idle instances only
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i0 = tb.x * 16;
     const int kr = lane >> 4, lc = lane & 15;
@@ -401,6 +415,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
     const int a0 = tb.x * TM;
     if (a0 >= nk) return;  // block-uniform
     const int n0 = tb.y * TN;
+    if (!kin_cols_active(ref, n0, TN, C, n_orb)) return;  // idle instances only
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
@@ -516,6 +531,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
     const int nk = nkeep[l];
     const int i0 = tb.x * TM;
     const int n0 = tb.y * TN;
+    if (!kin_cols_active(ref, n0, TN, C, n_orb)) return;  // idle instances only
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int kr = lane >> 4, lc = lane & 15;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
@@ -641,6 +657,7 @@ struct AngArgs {
     const int32_t *l0;     // blocks left to k_kin_prep (ground-state component), or null
     int N, Nl, Nlx, Nth, h, hq, sS, sA, nblk, C, n_orb, n_field, step;
     int alias;  // mixed form: S/D live in the l-space tile's LDS (see k_angular)
+    const int32_t *active;  // [n_inst] 0: instance idle, its work-groups return at once
     double dt;
 };
 
@@ -790,6 +807,7 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     const int lg = xcd_linear();
     const int korb = lg % g.n_orb, rest = lg / g.n_orb;
     const int tile_i = rest % gridDim.x, inst = rest / gridDim.x;
+    if (g.active && !g.active[inst]) return;  // idle serving slot (block-uniform)
     const int c = inst * g.n_orb + korb;
     const int i0 = tile_i * R;
     const int m = g.mcol[c % g.n_orb];
@@ -1097,7 +1115,8 @@ __global__ __launch_bounds__(WG) void k_rho(const T *__restrict__ dens,
                                             const double *__restrict__ wr,
                                             const double *__restrict__ omega,
                                             double *__restrict__ rho_u, double *__restrict__ vxc,
-                                            int N, int Nth, int n_orb, ObsArgs ob) {
+                                            int N, int Nth, int n_orb, ObsArgs ob,
+                                            const int32_t *__restrict__ active) {
     // the blocks past the (r, theta) grid (instance row 0) reduce the observable
     // partials of the step, one (column, observable) pair per wave
     const int nrho = (int)(((size_t)N * Nth + WG - 1) / WG);
@@ -1107,6 +1126,7 @@ __global__ __launch_bounds__(WG) void k_rho(const T *__restrict__ dens,
         return;
     }
     const int inst = blockIdx.y;
+    if (active && !active[inst]) return;  // idle serving slot (block-uniform)
     const size_t idx = (size_t)blockIdx.x * WG + threadIdx.x;
     if (idx >= (size_t)N * Nth) return;
     const int i = (int)(idx / Nth), j = (int)(idx % Nth);
@@ -1128,12 +1148,14 @@ struct GemmOperands {
     const double *A, *B;
     long sam, sak, saz, sbk, sbn, sbz;
     int M, N, K;
+    const int32_t *active;  // [z] 0: instance idle, skipped (null: all)
 };
 template <class Epi>
 __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
     __shared__ double red[3][64][4];
     const tile3 tb = xcd_tile();  // n tiles of one m tile run on one XCD and share A in its L2
     const int z = tb.z;
+    if (g.active && !g.active[z]) return;  // block-uniform
     const int m0 = tb.y * 16, n0 = tb.x * 16;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kr = lane >> 4, lc = lane & 15;
@@ -1222,10 +1244,12 @@ static __device__ void affine_scan(double &al, double &be, int dir, double (*sm)
 // (a < N - 1), hp[2 n_har][a] = 1/r_a
 __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
                                                 const double *__restrict__ hp,
-                                                double *__restrict__ V, int N, int n_har) {
+                                                double *__restrict__ V, int N, int n_har,
+                                                const int32_t *__restrict__ active) {
     __shared__ double sm[WG / 64][2];
     __shared__ double edge[2][WG];
     const int L = blockIdx.x, inst = blockIdx.y;
+    if (active && !active[inst]) return;  // idle serving slot (block-uniform)
     const int t = threadIdx.x;
     const double *qL = q + ((size_t)inst * n_har + L) * N;
     double *VL = V + ((size_t)inst * n_har + L) * N;
@@ -1530,6 +1554,7 @@ struct ggb_ctx {
     // device buffers
     double *psi, *Y, *Q, *QT, *B, *PL, *PLT, *chi, *omega, *r, *hpow, *wr, *v_gs, *mask, *occ, *rho_frozen;
     int32_t *nkeep, *mcol;
+    int32_t *active;  // [n_inst] instances in use (ggb_set_active), all at creation
     double2 *kph;  // [n_l][nk_max] (cos, sin)(lam dt) of the kinetic propagator
     double *dens, *vxc, *rho_u, *q, *V, *dv, *obs_part, *obs, *E;
     double2 *phase;
@@ -1598,7 +1623,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
                     ctx->r, ctx->hpow, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
-                    ctx->l0, ctx->ab, ctx->psi32};
+                    ctx->l0, ctx->ab, ctx->psi32, ctx->active};
     for (void *b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
@@ -1671,7 +1696,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->kph = nullptr;
     ctx->psi = ctx->Y = ctx->Q = ctx->QT = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
     ctx->r = ctx->hpow = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
-    ctx->nkeep = ctx->mcol = nullptr;
+    ctx->nkeep = ctx->mcol = ctx->active = nullptr;
     ctx->dens = ctx->vxc = ctx->rho_u = ctx->q = ctx->V = ctx->dv = ctx->obs_part = ctx->obs = ctx->E = nullptr;
     ctx->phase = nullptr;
     ctx->kin_variant = GGB_KIN_DEFAULT;
@@ -1722,6 +1747,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ok &= dalloc(&ctx->mask, N) == hipSuccess;
     ok &= dalloc(&ctx->occ, (size_t)p->n_orb) == hipSuccess;
     ok &= dalloc(&ctx->mcol, (size_t)p->n_orb) == hipSuccess;
+    ok &= dalloc(&ctx->active, (size_t)p->n_inst) == hipSuccess;
     ok &= dalloc(&ctx->dens, C * N * Nth) == hipSuccess;
     ok &= dalloc(&ctx->vxc, nI * N * Nth) == hipSuccess;
     ok &= dalloc(&ctx->rho_u, nI * N * Nth) == hipSuccess;
@@ -1746,6 +1772,11 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
         hipMemset(ctx->psi, 0, Nl * C * N * 2 * sizeof(double)) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess)
         return bail("ggb_create: hipMemset failed", GGB_E_HIP);
+    {
+        const std::vector<int32_t> all(p->n_inst, 1);  // every instance in use
+        if (hipMemcpy(ctx->active, all.data(), all.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return bail("ggb_create: copy of the instance mask failed", GGB_E_HIP);
+    }
     *out = ctx;
     return GGB_OK;
 }
@@ -1923,6 +1954,7 @@ static AngArgs ang_args(ggb_ctx *ctx, int step) {
     g.cur = nullptr;
     g.psi32 = nullptr;
     g.l0 = nullptr;
+    g.active = ctx->active;
     return g;
 }
 
@@ -1990,10 +2022,10 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
         const dim3 gr(nrho + nobs, p.n_inst);
         if (p.precision)
             k_rho<float><<<gr, WG, 0, st>>>(reinterpret_cast<const float *>(ctx->dens), ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
-                                            ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb, oa);
+                                            ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb, oa, ctx->active);
         else
             k_rho<double><<<gr, WG, 0, st>>>(ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
-                                             ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb, oa);
+                                             ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb, oa, ctx->active);
         GemmOperands gq;  // q[L][i] = sum_j PLT[L][j] rho_u[i][j]
         gq.A = ctx->PLT;
         gq.sam = Nth;
@@ -2004,11 +2036,12 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
         gq.sbn = Nth;
         gq.sbz = (long)N * Nth;
         gq.M = nh;
+        gq.active = ctx->active;
         gq.N = N;
         gq.K = Nth;
         QmomEpi eq{ctx->q, N, nh};
         k_tile_gemm<<<dim3((N + 15) / 16, (nh + 15) / 16, p.n_inst), WG, 0, st>>>(gq, eq);
-        k_hartree<<<dim3(nh, p.n_inst), WG, 0, st>>>(ctx->q, ctx->hpow, ctx->V, N, nh);
+        k_hartree<<<dim3(nh, p.n_inst), WG, 0, st>>>(ctx->q, ctx->hpow, ctx->V, N, nh, ctx->active);
     }
     GemmOperands gv;  // V(i, theta_j) = sum_L V[L][i] PLT[L][j]
     gv.A = ctx->V;
@@ -2020,6 +2053,7 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
     gv.sbn = 1;
     gv.sbz = 0;
     gv.M = N;
+    gv.active = ctx->active;
     gv.N = Nth;
     gv.K = p.interacting ? nh : 0;
     DvEpi ev{ctx->vxc, ctx->v_gs, ctx->mask, ctx->dv, ctx->phase, N, Nth, p.interacting, p.precision, p.dt};
@@ -2066,6 +2100,7 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh, hipEvent_t e0 =
     ref.phi = (mfma_t<T>::mixed && ctx->have_ref) ? ctx->phi_ref : nullptr;
     ref.l0 = ctx->l0;
     ref.ab = ctx->ab;
+    ref.active = ctx->active;
     if (pass == 0 && ref.phi)
         k_kin_ref<<<ctx->C, WG, 0, ctx->stream>>>(ctx->psi, ctx->phi_ref, ctx->l0, ctx->eps, ctx->ab,
                                                   p.n_r, ctx->C, p.n_orb, p.dt);
@@ -2181,6 +2216,19 @@ extern "C" int ggb_upload_columns(ggb_ctx *ctx, int32_t c0, int32_t nc, const do
     const size_t row = (size_t)p.n_r * 16;  // one column of one l
     HIPCHK(ctx, hipMemcpy2D(ctx->psi + (size_t)c0 * p.n_r * 2, ctx->C * row, psi, nc * row, nc * row,
                             p.n_l, hipMemcpyHostToDevice));
+    return GGB_OK;
+}
+
+extern "C" int ggb_set_active(ggb_ctx *ctx, const int32_t *active) {
+    if (!ctx) return GGB_E_ARG;
+    if (!active) return fail(ctx, GGB_E_ARG, "ggb_set_active: null mask");
+    const int n = ctx->p.n_inst;
+    std::vector<int32_t> m(n);
+    for (int k = 0; k < n; ++k) m[k] = active[k] ? 1 : 0;
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    // stream-ordered: work already queued sees the old mask, later work the new
+    HIPCHK(ctx, hipMemcpyAsync(ctx->active, m.data(), (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return GGB_OK;
 }
 

@@ -129,6 +129,13 @@ class Propagator:
                     "ggb_upload_columns")
         self._check(self.lib.ggb_rebuild_potential(self.ctx), "ggb_rebuild_potential")
 
+    def set_active(self, active):
+        """Mark the instance slots in use (bool per instance); idle ones skip
+        the angular and potential kernels."""
+        m = np.ascontiguousarray(np.asarray(active, dtype=bool).astype(np.int32))
+        assert m.shape == (self.n_inst,)
+        self._check(self.lib.ggb_set_active(self.ctx, _lib.iptr(m)), "ggb_set_active")
+
     def run(self, step0, nsteps, observables=True):
         """Propagate nsteps steps; returns dict(dip, acc, norm) of shape
         [nsteps, n_inst, n_orb] (None values when observables=False)."""

@@ -100,6 +100,13 @@ class Lane:
         self.step = 0
         self.active = [None] * slots
         self.orbital_steps = 0
+        # idle slots skip the angular and potential kernels
+        self._masked = hasattr(self.prop, "set_active")
+        self._mark()
+
+    def _mark(self):
+        if self._masked:
+            self.prop.set_active([a is not None for a in self.active])
 
     def free_slots(self):
         return [s for s, a in enumerate(self.active) if a is None]
@@ -110,11 +117,12 @@ class Lane:
         if n + self.chunk > self.ring:
             raise ValueError("request of %d steps does not fit the field ring (%d)" % (n, self.ring))
         E = p.midpoints(req.dt, n)
+        self.active[slot] = _Active(req=req, slot=slot, start=self.step, n_steps=n, E_end=None)
+        self._mark()  # in use before the potential of the new orbitals is built
         self.prop.upload_instance(slot)
         # field for the request's own steps, zero for the chunk tail after it
         self.prop.set_field_slice(slot, self.step, np.concatenate([E, np.zeros(self.chunk)]))
-        E_end = p((np.arange(n) + 1.0) * req.dt)
-        self.active[slot] = _Active(req=req, slot=slot, start=self.step, n_steps=n, E_end=E_end)
+        self.active[slot].E_end = p((np.arange(n) + 1.0) * req.dt)
         return n
 
     def advance(self):
@@ -133,6 +141,8 @@ class Lane:
             if a.done >= a.n_steps:
                 finished.append(a)
                 self.active[s] = None
+        if finished:
+            self._mark()
         return finished
 
     def busy(self):

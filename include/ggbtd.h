@@ -131,6 +131,14 @@ int ggb_set_field_slice(ggb_ctx *ctx, int32_t inst, int32_t start, int32_t n, co
  * h_initialiseTDKSOSet (main.cu:463). */
 int ggb_upload_columns(ggb_ctx *ctx, int32_t c0, int32_t nc, const double *psi);
 
+/* Mark which instances are in use (active[n_inst], nonzero = in use; all are
+ * at creation).  The angular, density and potential kernels skip idle
+ * instances (their columns keep stale values; the kinetic products still run
+ * over every column), so a serving lane with free slots does less work.  A
+ * slot being seated must be marked in use before ggb_rebuild_potential.  No
+ * counterpart in the reference, which runs its scans one after another. */
+int ggb_set_active(ggb_ctx *ctx, const int32_t *active);
+
 /* Rebuild the potential of every instance from the current orbitals.  For
  * instances whose orbitals did not change this reproduces the potential
  * they already hold, bit for bit. */

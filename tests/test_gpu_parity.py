@@ -265,3 +265,28 @@ def test_timing_modes(precision, n_inst):
     for o in outs[1:]:
         for key in ("dip", "acc", "norm"):
             assert np.array_equal(o[key], outs[0][key]), key
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_idle_instances_skipped(precision):
+    """Instances marked idle (ggb_set_active) skip the angular and potential
+    kernels; the instances in use give exactly the results of a run with every
+    instance in use."""
+    _say("case: idle instance slots, Ne N=48 Lmax=6, 3 instances, precision %d" % precision)
+    dt = 0.05
+    S = AtomSystem("Ne", 48, 30.0, 6, dt)
+    t = (np.arange(20) + 0.5) * dt
+    fields = np.stack([0.05 * np.sin(0.4 * t), -0.08 * np.cos(0.3 * t), 0.02 + 0.0 * t])
+    eng = _engine()
+    runs = []
+    for mask in ([1, 1, 1], [1, 0, 1]):
+        with eng.Propagator(S, n_inst=3, precision=precision) as prop:
+            prop.set_field(fields)
+            prop.set_active(mask)
+            runs.append((prop.run(0, 20), prop.download_orbitals()))
+    (full, psi_full), (part, psi_part) = runs
+    for key in ("dip", "acc", "norm"):
+        assert np.array_equal(full[key][:, [0, 2]], part[key][:, [0, 2]]), key
+    for k in (0, 2):
+        cols = slice(k * S.n_orb, (k + 1) * S.n_orb)
+        assert np.array_equal(psi_full[:, cols], psi_part[:, cols])
