@@ -2214,8 +2214,11 @@ extern "C" int ggb_upload_columns(ggb_ctx *ctx, int32_t c0, int32_t nc, const do
     HIPCHK(ctx, hipSetDevice(p.device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     const size_t row = (size_t)p.n_r * 16;  // one column of one l
-    HIPCHK(ctx, hipMemcpy2D(ctx->psi + (size_t)c0 * p.n_r * 2, ctx->C * row, psi, nc * row, nc * row,
-                            p.n_l, hipMemcpyHostToDevice));
+    // stream-ordered copies only: a synchronous copy on the legacy stream
+    // fails while another thread captures a graph on its own stream
+    HIPCHK(ctx, hipMemcpy2DAsync(ctx->psi + (size_t)c0 * p.n_r * 2, ctx->C * row, psi, nc * row,
+                                 nc * row, p.n_l, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return GGB_OK;
 }
 
@@ -2276,7 +2279,8 @@ extern "C" int ggb_field_ring(ggb_ctx *ctx, int32_t capacity) {
         ctx->E = nullptr;
     }
     HIPCHK(ctx, dalloc(&ctx->E, (size_t)p.n_inst * capacity));
-    HIPCHK(ctx, hipMemset(ctx->E, 0, (size_t)p.n_inst * capacity * 8));
+    HIPCHK(ctx, hipMemsetAsync(ctx->E, 0, (size_t)p.n_inst * capacity * 8, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->n_field = capacity;
     ctx->field_ring = true;
     return GGB_OK;
@@ -2297,8 +2301,11 @@ extern "C" int ggb_set_field_slice(ggb_ctx *ctx, int32_t inst, int32_t start, in
     double *base = ctx->E + (size_t)inst * ctx->n_field;
     const int s0 = start % ctx->n_field;
     const int first = (s0 + n <= ctx->n_field) ? n : ctx->n_field - s0;
-    HIPCHK(ctx, hipMemcpy(base + s0, E, (size_t)first * 8, hipMemcpyHostToDevice));
-    if (first < n) HIPCHK(ctx, hipMemcpy(base, E + first, (size_t)(n - first) * 8, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpyAsync(base + s0, E, (size_t)first * 8, hipMemcpyHostToDevice, ctx->stream));
+    if (first < n)
+        HIPCHK(ctx, hipMemcpyAsync(base, E + first, (size_t)(n - first) * 8, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return GGB_OK;
 }
 
@@ -2418,7 +2425,8 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
     }
     if (dip || acc || norm) {
         double *h = new double[need];
-        hipError_t e = hipMemcpy(h, ctx->obs, need * 8, hipMemcpyDeviceToHost);
+        hipError_t e = hipMemcpyAsync(h, ctx->obs, need * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
             delete[] h;
             return fail(ctx, GGB_E_HIP, std::string("ggb_run: copy of observables: ") + hipGetErrorString(e));
@@ -2438,8 +2446,9 @@ extern "C" int ggb_download_orbitals(ggb_ctx *ctx, double *psi) {
     if (!psi) return fail(ctx, GGB_E_ARG, "ggb_download_orbitals: null psi");
     HIPCHK(ctx, hipSetDevice(ctx->p.device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    HIPCHK(ctx, hipMemcpy(psi, ctx->psi, (size_t)ctx->p.n_l * ctx->C * ctx->p.n_r * 16,
-                          hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpyAsync(psi, ctx->psi, (size_t)ctx->p.n_l * ctx->C * ctx->p.n_r * 16,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return GGB_OK;
 }
 
@@ -2448,8 +2457,9 @@ extern "C" int ggb_download_potential(ggb_ctx *ctx, double *dv) {
     if (!dv) return fail(ctx, GGB_E_ARG, "ggb_download_potential: null buffer");
     HIPCHK(ctx, hipSetDevice(ctx->p.device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    HIPCHK(ctx, hipMemcpy(dv, ctx->dv, (size_t)ctx->p.n_inst * ctx->p.n_r * ctx->p.n_theta * 8,
-                          hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpyAsync(dv, ctx->dv, (size_t)ctx->p.n_inst * ctx->p.n_r * ctx->p.n_theta * 8,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return GGB_OK;
 }
 

@@ -168,7 +168,11 @@ int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step);
 /* Replay the step loop of ggb_run from a hipGraph of `steps` steps (0 =
  * plain stream launches, the default).  The graph is captured on the next
  * ggb_run and re-captured after any call that may move a buffer; runs with
- * timing enabled use plain launches.  No counterpart in the reference. */
+ * timing enabled use plain launches.  Calls on one context copy on that
+ * context's stream only, so other threads may drive other contexts during a
+ * capture; ggb_create, ggb_upload_static, ggb_spectrum, ggb_ground_state and
+ * ggb_destroy allocate, free or use the legacy stream, and must not overlap
+ * a capture in another thread.  No counterpart in the reference. */
 int ggb_set_graph(ggb_ctx *ctx, int32_t steps);
 
 /* Kinetic-pass kernel variants (tile shapes).  DEFAULT: NARROW up to 8
