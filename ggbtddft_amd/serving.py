@@ -117,12 +117,17 @@ class Lane:
         if n + self.chunk > self.ring:
             raise ValueError("request of %d steps does not fit the field ring (%d)" % (n, self.ring))
         E = p.midpoints(req.dt, n)
-        self.active[slot] = _Active(req=req, slot=slot, start=self.step, n_steps=n, E_end=None)
-        self._mark()  # in use before the potential of the new orbitals is built
-        self.prop.upload_instance(slot)
-        # field for the request's own steps, zero for the chunk tail after it
-        self.prop.set_field_slice(slot, self.step, np.concatenate([E, np.zeros(self.chunk)]))
-        self.active[slot].E_end = p((np.arange(n) + 1.0) * req.dt)
+        E_end = p((np.arange(n) + 1.0) * req.dt)
+        self.active[slot] = _Active(req=req, slot=slot, start=self.step, n_steps=n, E_end=E_end)
+        try:
+            self._mark()  # in use before the potential of the new orbitals is built
+            self.prop.upload_instance(slot)
+            # field for the request's own steps, zero for the chunk tail after it
+            self.prop.set_field_slice(slot, self.step, np.concatenate([E, np.zeros(self.chunk)]))
+        except Exception:
+            self.active[slot] = None  # the slot stays free
+            self._mark()
+            raise
         return n
 
     def advance(self):
