@@ -39,7 +39,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(system_args, E, nsteps=40, warm=3, nsteps_1core=10):
+def cpu_baseline(system_args, E, nsteps=100, warm=3, nsteps_1core=40):
     """FP64 oracle on the host for a bounded window (setup excluded): all
     cores (BLAS threads = min(16, nproc)), then one core on a shorter window."""
     import contextlib
@@ -163,6 +163,78 @@ def serving_run(sysm_ar, args):
     return stats
 
 
+def scan_requests(n, seed=0):
+    """The batched scan: n instances from `seed` over He and Ne (uniform),
+    intensity log-uniform in [5e13, 5e14] W/cm^2, wavelength uniform in
+    [600, 2000] nm, sin^2 length uniform integer in [4, 16] cycles (uneven step
+    counts); the grid from the quiver radius alpha = E0 / omega0^2: N = 256,
+    L_max = 32, r_max = 100 below 25 a.u., N = 384, L_max = 48, r_max = 150
+    below 60 a.u., else N = 512, L_max = 64, r_max = 200; dt = 0.05."""
+    from ggbtddft_amd.pulse import Pulse
+    from ggbtddft_amd.serving import Request
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(n):
+        atom = "He" if rng.uniform() < 0.5 else "Ne"
+        I = float(np.exp(rng.uniform(np.log(5e13), np.log(5e14))))
+        lam = float(rng.uniform(600.0, 2000.0))
+        cyc = int(rng.integers(4, 17))
+        p = Pulse(I, lam, cyc)
+        alpha = p.E0 / p.omega0 ** 2
+        nr, lmax, rmax = ((256, 32, 100.0) if alpha < 25.0 else
+                          (384, 48, 150.0) if alpha < 60.0 else (512, 64, 200.0))
+        reqs.append(Request(rid=k, atom=atom, intensity=I, wavelength=lam, n_cycles=cyc, N=nr,
+                            r_max=rmax, lmax=lmax, dt=0.05))
+    return reqs
+
+
+def scan_run(n):
+    """The batched scan of n instances served as one batch (all requests at
+    t = 0, one lane per grid key, each lane with a slot per request of its
+    key, idle slots skipped once their request is done), FP64 then FP32 mixed;
+    orbital-timesteps/s over the whole scan and the FP32 error per instance."""
+    from ggbtddft_amd.serving import Server
+    reqs = scan_requests(n)
+    keys = {}
+    for r in reqs:
+        keys[r.key()] = keys.get(r.key(), 0) + 1
+    slots = max(keys.values())
+    n_steps = [r.pulse().n_steps(r.dt) for r in reqs]
+    t0 = time.perf_counter()
+    systems = {}
+    for key in keys:   # ground states and tables before the clock starts
+        systems[key] = Server(slots=1)._system(key)
+    t_sys = time.perf_counter() - t0
+    out = {"instances": n, "lanes": len(keys), "slots_per_lane": slots,
+           "steps_min": int(min(n_steps)), "steps_max": int(max(n_steps)),
+           "system_setup_s_before_clock": t_sys,
+           "workload": "batched scan (seed 0): He/Ne, 5e13-5e14 W/cm^2 log-uniform, "
+                       "600-2000 nm, 4-16 cycles, grid by quiver radius "
+                       "(N 256/384/512, L_max 32/48/64), dt 0.05, all requests at t = 0"}
+    res = {}
+    for prec, name in ((0, "fp64"), (1, "fp32")):
+        log("scan %s: serving %d requests on %d lanes" % (name, n, len(keys)))
+        srv = Server(slots=slots, chunk=256, ring=1 << 17, precision=prec)
+        srv.systems.update(systems)
+        results, stats = srv.serve(reqs, realtime=False)
+        srv.close()
+        res[name] = results
+        out[name] = {"wall_s": stats["wall_s"], "served": stats["served"],
+                     "orbital_timesteps_per_s": stats["orbital_timesteps_per_s"],
+                     "device_bytes": stats["device_bytes"]}
+        log("scan %s: %d served in %.1f s, %.0f orbital-timesteps/s" % (
+            name, stats["served"], stats["wall_s"], stats["orbital_timesteps_per_s"]))
+    err = []
+    for a, b in zip(res["fp64"], res["fp32"]):
+        if a.error is None and b.error is None:
+            err.append(float(np.linalg.norm(b.a_total - a.a_total) / np.linalg.norm(a.a_total)))
+    out["ratio_fp32_vs_fp64"] = out["fp32"]["orbital_timesteps_per_s"] / \
+        out["fp64"]["orbital_timesteps_per_s"]
+    out["fp32_acc_rel_l2_median"] = float(np.median(err)) if err else None
+    out["fp32_acc_rel_l2_max"] = float(np.max(err)) if err else None
+    return out
+
+
 def spectrum_timing(nfull):
     """Device vs host (numpy) spectrum of one full-length Ar acceleration
     series (window, padding to 17 n, FFT, |.|^2, 20 harmonic masks)."""
@@ -283,7 +355,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--scan", type=int, default=0, metavar="N",
+                    help="also serve the batched He/Ne scan of N instances, FP64 then FP32 "
+                         "(64 is the scan configuration; minutes of GPU time)")
     ap.add_argument("--batched", type=int, default=64,
                     help="Ar instances for the batched FP32 leg (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=200)
@@ -395,6 +470,8 @@ def main():
         line["batched_fp32"] = batched_run(sysm, pulse, args, value)
     if args.serving > 0:
         line["serving"] = serving_run(sysm, args)
+    if args.scan > 0:
+        line["scan"] = scan_run(args.scan)
     line["spectrum"] = spectrum_timing(nfull)
     line["ground_state"] = gs_stats
     if not args.no_precision_report:
