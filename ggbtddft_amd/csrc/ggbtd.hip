@@ -219,7 +219,10 @@ static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, in
 // so that pass 2 reads contiguous runs of a.  T is the compute type (tables
 // and Y in T, state in FP64, phase in FP64).
 // ---------------------------------------------------------------------------
-#define KU 8
+#ifndef KIN_KU
+#define KIN_KU 8
+#endif
+#define KU KIN_KU
 template <typename T>
 __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 const double2 *__restrict__ kph,
