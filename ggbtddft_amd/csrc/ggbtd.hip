@@ -1620,19 +1620,19 @@ extern "C" const char *ggb_last_error(const ggb_ctx *ctx) {
 
 extern "C" void ggb_destroy(ggb_ctx *ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->p.device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->p.device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->psi, ctx->Y, ctx->Q, ctx->QT, ctx->kph, ctx->B, ctx->PL, ctx->PLT, ctx->chi, ctx->omega,
                     ctx->r, ctx->hpow, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
                     ctx->l0, ctx->ab, ctx->psi32, ctx->active};
     for (void *b : bufs)
-        if (b) hipFree(b);
-    for (hipEvent_t e : ctx->events) hipEventDestroy(e);
-    if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
-    if (ctx->dcur) hipFree(ctx->dcur);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->dcur) (void)hipFree(ctx->dcur);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -1919,7 +1919,7 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
         HIPCHK(ctx, hipMemcpyAsync(ctx->rho_frozen, s->rho_frozen, N * 8, hipMemcpyHostToDevice, st));
     } else if (ctx->rho_frozen) {
         HIPCHK(ctx, hipStreamSynchronize(st));
-        hipFree(ctx->rho_frozen);
+        (void)hipFree(ctx->rho_frozen);
         ctx->rho_frozen = nullptr;
     }
     HIPCHK(ctx, hipStreamSynchronize(st));
@@ -2258,7 +2258,7 @@ extern "C" int ggb_set_field(ggb_ctx *ctx, const double *E, int32_t n_steps) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     drop_graph(ctx);
     if (ctx->E) {
-        hipFree(ctx->E);
+        (void)hipFree(ctx->E);
         ctx->E = nullptr;
     }
     HIPCHK(ctx, dalloc(&ctx->E, (size_t)p.n_inst * n_steps));
@@ -2278,7 +2278,7 @@ extern "C" int ggb_field_ring(ggb_ctx *ctx, int32_t capacity) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     drop_graph(ctx);
     if (ctx->E) {
-        hipFree(ctx->E);
+        (void)hipFree(ctx->E);
         ctx->E = nullptr;
     }
     HIPCHK(ctx, dalloc(&ctx->E, (size_t)p.n_inst * capacity));
@@ -2367,7 +2367,7 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
     if (need > ctx->obs_cap) {
         drop_graph(ctx);
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->obs) hipFree(ctx->obs);
+        if (ctx->obs) (void)hipFree(ctx->obs);
         ctx->obs = nullptr;
         ctx->obs_cap = 0;
         HIPCHK(ctx, dalloc(&ctx->obs, need));
@@ -2504,7 +2504,7 @@ extern "C" int ggb_selftest_mfma(int32_t device, double *out) {
     k_selftest_mfma<<<1, 64>>>(d);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(out, d, 512 * sizeof(double), hipMemcpyDeviceToHost);
-    hipFree(d);
+    (void)hipFree(d);
     return e == hipSuccess ? GGB_OK : GGB_E_HIP;
 }
 

@@ -95,14 +95,19 @@ class Lane:
         if hasattr(self.prop, "autotune"):  # kinetic tile shapes for this lane's shape
             self.prop.autotune(reps=3)
         self.prop.field_ring(ring)
-        if hasattr(self.prop, "set_graph"):  # each chunk replays a captured run of steps
-            self.prop.set_graph(min(64, chunk))
         self.step = 0
         self.active = [None] * slots
         self.orbital_steps = 0
         # idle slots skip the angular and potential kernels
         self._masked = hasattr(self.prop, "set_active")
         self._mark()
+        if hasattr(self.prop, "set_graph"):  # each chunk replays a captured run of steps
+            self.prop.set_graph(min(64, chunk))
+            # One idle chunk here, in the creating thread while no lane runs,
+            # captures the graph and sizes the observable buffer, so the
+            # chunks run from the lane threads neither capture nor allocate.
+            self.prop.run(0, chunk)
+            self.step = chunk
 
     def _mark(self):
         if self._masked:
