@@ -72,7 +72,8 @@ def cpu_baseline(system_args, E, nsteps=100, warm=3, nsteps_1core=40):
                        "steps, %.1f s (setup %.1f s excluded)" % (k, warm, el, setup),
                 value_1core=v1,
                 sample_1core="same, one thread, %d steps, %.1f s" % (k1, el1),
-                host_cpu=_cpu_model())
+                host_cpu=_cpu_model(), host_cpus_visible=os.cpu_count(),
+                host_cpus_usable=len(os.sched_getaffinity(0)))
 
 
 def _cpu_model():
