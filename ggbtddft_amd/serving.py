@@ -92,22 +92,27 @@ class Lane:
     def __init__(self, system, slots, chunk, ring, factory, device=0):
         self.system, self.slots, self.chunk, self.ring = system, slots, chunk, ring
         self.prop = factory(system, n_inst=slots, device=device)
-        if hasattr(self.prop, "autotune"):  # kinetic tile shapes for this lane's shape
-            self.prop.autotune(reps=3)
-        self.prop.field_ring(ring)
         self.step = 0
         self.active = [None] * slots
         self.orbital_steps = 0
         # idle slots skip the angular and potential kernels
         self._masked = hasattr(self.prop, "set_active")
-        self._mark()
-        if hasattr(self.prop, "set_graph"):  # each chunk replays a captured run of steps
-            self.prop.set_graph(min(64, chunk))
-            # One idle chunk here, in the creating thread while no lane runs,
-            # captures the graph and sizes the observable buffer, so the
-            # chunks run from the lane threads neither capture nor allocate.
-            self.prop.run(0, chunk)
-            self.step = chunk
+        try:
+            if hasattr(self.prop, "autotune"):  # kinetic tile shapes for this lane's shape
+                self.prop.autotune(reps=3)
+            self.prop.field_ring(ring)
+            self._mark()
+            if hasattr(self.prop, "set_graph"):  # each chunk replays a captured run of steps
+                self.prop.set_graph(min(64, chunk))
+                # One idle chunk here, in the creating thread while no lane runs,
+                # captures the graph and sizes the observable buffer, so the
+                # chunks run from the lane threads neither capture nor allocate.
+                self.prop.run(0, chunk)
+                self.step = chunk
+        except Exception:
+            if hasattr(self.prop, "close"):  # no half-made lane keeps device memory
+                self.prop.close()
+            raise
 
     def _mark(self):
         if self._masked:
@@ -185,7 +190,6 @@ class Server:
         # spectra on the GPU with the real propagator; the host definition
         # (same numbers, numpy FFT) with a stand-in backend
         self.device_spectrum = factory is None
-        self.lock = threading.Lock()
 
     def _system(self, key):
         if key not in self.systems:
@@ -279,13 +283,21 @@ class Server:
                 th.join()
             for lane, out in zip(busy, outs):
                 for item in out:
-                    if isinstance(item, Exception):
+                    if isinstance(item, Exception):  # the lane's requests fail together
                         for a in lane.active:
                             if a is not None:
                                 errors[a.req.rid] = repr(item)
                         lane.active = [None] * lane.slots
+                        try:
+                            lane._mark()
+                        except Exception:
+                            pass
                         continue
-                    res = self._finish(lane, item, t0)
+                    try:
+                        res = self._finish(lane, item, t0)
+                    except Exception as exc:  # e.g. the spectrum: this request fails alone
+                        errors[item.req.rid] = repr(exc)
+                        continue
                     res.t_admit = admit_time.get(item.req.rid, math.nan)
                     results[item.req.rid] = res
         out = []

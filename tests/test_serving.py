@@ -1,6 +1,7 @@
 """Serving loop (continuous batching) on the CPU with the oracle-backed
 stand-in propagator: slot reuse, ring-buffer field offsets, per-request
-truncation and spectra match a direct oracle run of each request."""
+truncation and spectra match a direct oracle run of each request; a bad
+request or a failing lane fails only its own requests."""
 import numpy as np
 
 from fake_backend import OraclePropagator
@@ -42,3 +43,34 @@ def test_budget_too_small_fails_cleanly():
     srv.footprint = lambda sysm: 1
     results, stats = srv.serve(reqs, realtime=False)
     assert stats["failed"] == 1 and "budget" in results[0].error
+
+
+def test_request_too_long_for_the_ring_fails_alone():
+    reqs = _requests()[:3]
+    long_req = Request(rid=99, atom="He", intensity=1e14, wavelength=800, n_cycles=40, N=24,
+                       r_max=20.0, lmax=3, dt=0.2)
+    srv = Server(slots=2, chunk=40, ring=2048, factory=OraclePropagator)
+    results, stats = srv.serve(reqs + [long_req], realtime=False)
+    assert stats["served"] == 3 and stats["failed"] == 1
+    assert "field ring" in results[-1].error
+    assert all(r.error is None for r in results[:3])
+
+
+class _FailingPropagator(OraclePropagator):
+    """Stand-in whose chunk fails for one atom's lane (a device error)."""
+
+    def run(self, step0, nsteps):
+        if self.sys.atom.symbol == "Ne":
+            raise RuntimeError("device error in this lane")
+        return super().run(step0, nsteps)
+
+
+def test_failing_lane_fails_only_its_requests():
+    reqs = _requests()[:2]
+    reqs.append(Request(rid=7, atom="Ne", intensity=1e14, wavelength=800, n_cycles=1, N=24,
+                        r_max=20.0, lmax=3, dt=0.2))
+    srv = Server(slots=2, chunk=40, ring=2048, factory=_FailingPropagator)
+    results, stats = srv.serve(reqs, realtime=False)
+    assert stats["served"] == 2 and stats["failed"] == 1
+    assert "device error" in results[-1].error
+    assert all(r.error is None for r in results[:2])
