@@ -157,7 +157,8 @@ class Propagator:
 
     def autotune(self, reps=10, cache=True):
         """Pick the fastest kinetic variant for this context's shape (timed on
-        the device, orbitals restored); remembered per shape in this process.
+        the device with the current instance mask, orbitals restored);
+        remembered per shape in this process unless cache is False.
         Returns (name, {name: ms per kinetic step})."""
         key = (self.sys.N, self.sys.n_l, self.sys.nk_max, self.C, self.precision)
         if cache and key in _TUNED:
@@ -170,7 +171,8 @@ class Propagator:
                     "ggb_autotune")
         times = {_lib.KIN_VARIANTS[v]: float(ms[v]) for v in range(1, 4) if ms[v] > 0}
         name = _lib.KIN_VARIANTS[chosen.value]
-        _TUNED[key] = (name, times)
+        if cache:
+            _TUNED[key] = (name, times)
         return name, times
 
     def set_graph(self, steps):

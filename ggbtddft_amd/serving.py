@@ -95,11 +95,25 @@ class Lane:
         self.step = 0
         self.active = [None] * slots
         self.orbital_steps = 0
-        # idle slots skip the angular and potential kernels
+        # idle slots skip the angular and potential kernels (and kinetic tiles
+        # whose instances are all idle)
         self._masked = hasattr(self.prop, "set_active")
+        self._variant = None
+        self._variants = None  # {variant: (ms with one slot in use, ms with all)}
         try:
             if hasattr(self.prop, "autotune"):  # kinetic tile shapes for this lane's shape
-                self.prop.autotune(reps=3)
+                if self._masked:
+                    self.prop.set_active([True] * slots)
+                self._variant, t_all = self.prop.autotune(reps=3)
+                if self._masked and slots > 1 and hasattr(self.prop, "set_kinetic_variant"):
+                    # with few slots in use a narrower variant can win (it skips idle
+                    # columns at a finer grain); time each with one slot in use too
+                    self.prop.set_active([True] + [False] * (slots - 1))
+                    _, t_one = self.prop.autotune(reps=3, cache=False)
+                    self.prop.set_kinetic_variant(self._variant)
+                    keys = set(t_all) & set(t_one)
+                    if len(keys) > 1:
+                        self._variants = {k: (t_one[k], t_all[k]) for k in keys}
             self.prop.field_ring(ring)
             self._mark()
             if hasattr(self.prop, "set_graph"):  # each chunk replays a captured run of steps
@@ -115,8 +129,21 @@ class Lane:
             raise
 
     def _mark(self):
-        if self._masked:
-            self.prop.set_active([a is not None for a in self.active])
+        """Hand the slots in use to the device and, when the variants were timed
+        at one slot and at all, take the one whose linear interpolation in the
+        number of slots in use is fastest."""
+        if not self._masked:
+            return
+        mask = [a is not None for a in self.active]
+        self.prop.set_active(mask)
+        n = sum(mask)
+        if self._variants and n > 0:
+            f = (n - 1) / (self.slots - 1)
+            best = min(sorted(self._variants),
+                       key=lambda k: self._variants[k][0] + f * (self._variants[k][1] - self._variants[k][0]))
+            if best != self._variant:
+                self.prop.set_kinetic_variant(best)  # the next chunk captures its graph again
+                self._variant = best
 
     def free_slots(self):
         return [s for s, a in enumerate(self.active) if a is None]
