@@ -140,7 +140,7 @@ def serving_run(sysm_ar, args):
             second = None
         reqs.append(Request(rid=k, atom=atom, intensity=I, wavelength=lam, n_cycles=2, N=nr,
                             r_max=rmax, lmax=lmax, dt=dt, arrival=float(arr[k]), second=second))
-    srv = Server(slots=args.serving_slots, chunk=256, ring=1 << 16, precision=1)
+    srv = Server(slots=args.serving_slots, chunk=args.serving_chunk, ring=1 << 16, precision=1)
     srv.systems[(ATOM, N, float(R_MAX), LMAX, float(DT))] = sysm_ar
     t0 = time.perf_counter()
     for key in {r.key() for r in reqs}:   # ground states and tables before the clock starts
@@ -151,7 +151,8 @@ def serving_run(sysm_ar, args):
     bad = [r for r in results if r.error is not None or not np.all(np.isfinite(r.spectrum))]
     stats = dict(stats)
     stats.update({"requests": n, "arrival_rate_per_s": args.serving_rate,
-                  "slots_per_lane": args.serving_slots, "precision": "fp32 mixed",
+                  "slots_per_lane": args.serving_slots, "chunk_steps": args.serving_chunk,
+                  "precision": "fp32 mixed",
                   "workload": "Poisson stream (seed 0): He N=256 Lmax=32, Ne N=384 Lmax=48 "
                               "(dt=0.05) and Kr N=768 Lmax=96 (dt=0.02), one colour "
                               "800-2000 nm, 5e13-5e14 W/cm^2; Ar N=512 Lmax=64 (dt=0.02) "
@@ -367,6 +368,8 @@ def main():
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-slots", type=int, default=8)
+    ap.add_argument("--serving-chunk", type=int, default=256,
+                    help="steps per lane between seatings (a multiple of 64 keeps graph replay whole)")
     ap.add_argument("--no-precision-report", action="store_true",
                     help="skip the full-length FP64 / FP32 / batched-FP32 runs and their "
                          "harmonic-intensity errors")
