@@ -15,6 +15,10 @@ class OraclePropagator:
         self.dV = [self.O.potential(self.O.theta_state(p)) for p in self.psi]
         self.E = np.zeros((n_inst, 1))
 
+    def set_field(self, E):
+        E = np.asarray(E, dtype=np.float64)
+        self.E = np.tile(E, (self.n_inst, 1)) if E.ndim == 1 else E.copy()
+
     def field_ring(self, capacity):
         self.E = np.zeros((self.n_inst, capacity))
 
