@@ -50,3 +50,15 @@ def test_driver_matches_oracle(tmp_path):
     acc = np.fromfile(os.path.join(base, "timeDependent", "observables", "dipoleAcceleration",
                                    "scan002", "acc.bin"))
     assert np.array_equal(acc, scans[2]["acc_total"])
+
+
+def test_driver_fp32_mixed_close_to_fp64(tmp_path):
+    print("case: driver FP32 mixed vs FP64, He N=32 Lmax=5, 3 in-situ scans", flush=True)
+    cfg = RunConfig(read_init(INIT))
+    _, s64 = R.run(cfg, str(tmp_path / "o64"), precision=0)
+    _, s32 = R.run(cfg, str(tmp_path / "o32"), precision=1)
+    for a, b in zip(s64, s32):
+        rel = np.linalg.norm(b["acc_total"] - a["acc_total"]) / np.linalg.norm(a["acc_total"])
+        print("acc rel L2 fp32 vs fp64", rel, flush=True)
+        assert rel < 1e-4
+        assert np.abs(b["norm"] - a["norm"]).max() < 1e-5
