@@ -1,7 +1,7 @@
 """Imaginary-time FP64 ground state with Gram-Schmidt (precompute, host).
 
 Replaces the reference's diagonalisation SCF (src/scripts/phys/groundState.py:182-457)
-with the north-star method.  One radial function per (n, l) shell, seeded
+with an imaginary-time iteration.  One radial function per (n, l) shell, seeded
 complex128 normal start (default_rng(seed): real then imaginary N-vector per
 shell, shells in atom order).  Each iteration:
     v_out = V_0[rho_u] + v_xc[rho]          spherical Hartree + LDA

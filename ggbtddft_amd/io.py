@@ -1,7 +1,6 @@
 """Reference-compatible files (raw little-endian, numpy ``tofile`` order).
 
-Writers follow the layouts the reference's loaders and analysis read
-(SURVEY appendix A1):
+Writers follow the layouts the reference's loaders and analysis read:
 
 workingData/mesh/{r,P,drdx}.bin                f64 x N       (init.py:356-358 -> h_initialiseMesh.cuh)
 workingData/mesh/time.bin                      f64 x N_t     (init.py:359)

@@ -1,7 +1,7 @@
 """Static data of one atom on one grid: the precompute stage.
 
 Counterpart of src/scripts/init.py:304-337 (mesh, absorber, ground state,
-radial propagators, angular tables) with the north-star choices:
+radial propagators, angular tables) with this propagator's choices:
 * kinetic propagator exp(-i H_l dt), H_l = -T/2 + l(l+1)/2r^2 - Z/r + v_gs,
   kept in spectral form Q_l diag(e^{-i lam dt}) Q_l^T on the eigenpairs with
   lam <= E_cut = ecut_factor * pi / dt (components above turn by more than pi

@@ -8,7 +8,7 @@ GPU invocation) and the ``cpu_baseline`` leg of ``bench.py``.
 The oracle is a plain numpy restatement of the equations the GPU path
 implements.  Where the equations follow the reference (graham-g-brown/ggbTDDFT)
 each function cites the reference file:line it restates; where the equations
-are this project's own choice (the north-star scheme: 2x2 laser rotations,
+are this project's own choice (its scheme: 2x2 laser rotations,
 theta-space potentials, Hartree multipoles by radial scans, LDA Slater+PW92,
 imaginary-time ground state) the docstring says so and names the closed forms
 or published results that pin it.

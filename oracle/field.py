@@ -4,7 +4,7 @@ Unit conversions restate ``init.py:10-14, 255-256``: E0 = sqrt(I / 3.509e16)
 for I in W/cm^2; omega0 = 45.5634 / lambda[nm] (hbar c in eV nm over the
 Hartree, 1239.84193 / 27.2114).
 
-The north-star pulse is a sin^2 envelope (the reference uses a super-Gaussian,
+This propagator's pulse is a sin^2 envelope (the reference uses a super-Gaussian,
 ``h_initialiseField.cuh:20``):
     E(t) = E0 sin^2(pi t / tau) cos(omega0 (t - tau/2) + cep),  0 <= t <= tau,
     tau  = n_cycles * 2 pi / omega0,

@@ -1,6 +1,6 @@
 """Imaginary-time FP64 ground state (oracle; test infrastructure only).
 
-North-star method (the reference diagonalises instead, ``groundState.py:182-457``):
+Imaginary-time iteration (the reference diagonalises instead, ``groundState.py:182-457``):
 one radial function u_s per closed (n, l) shell, seeded complex128 normal start
 (numpy default_rng(seed), real then imaginary part per shell, shells in atom
 order), then repeated

@@ -6,7 +6,7 @@
 n_pad * N_t on each side (N_pad = 8, ``analysis.py:147``), then
 fftshift(fft(fftshift(.))) and |.|^2.
 
-``acceleration_spectrum`` is the north-star form from the Ehrenfest
+``acceleration_spectrum`` is this project's form from the Ehrenfest
 acceleration: same window and padding, no differentiation.
 
 ``harmonic_intensities`` restates the harmonic masks of

@@ -7,7 +7,7 @@ Correlation: Perdew & Wang, Phys. Rev. B 45, 13244 (1992), spin-unpolarised
 G(r_s; A, a1, b1..b4) with p = 1 and the published constants
 A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294.
 The reference uses Chachiyo correlation instead (``dft.py:58-87``); PW92 is the
-north-star choice.
+choice of this project.
 
   eps_c = -2A(1 + a1 rs) ln(1 + 1/(2A(b1 rs^1/2 + b2 rs + b3 rs^3/2 + b4 rs^2)))
   v_c   = eps_c - rs/3 d eps_c/d rs
