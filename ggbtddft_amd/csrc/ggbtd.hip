@@ -1130,11 +1130,23 @@ __global__ __launch_bounds__(WG) void k_rho(const T *__restrict__ dens,
     }
     const int inst = blockIdx.y;
     if (active && !active[inst]) return;  // idle serving slot (block-uniform)
-    const size_t idx = (size_t)blockIdx.x * WG + threadIdx.x;
-    if (idx >= (size_t)N * Nth) return;
-    const int i = (int)(idx / Nth), j = (int)(idx % Nth);
+    const int npt = N * Nth;  // < 2^31 (ggb_create bounds N and N_theta)
+    const int idx = (int)blockIdx.x * WG + (int)threadIdx.x;
+    if (idx >= npt) return;
+    const int i = idx / Nth, j = idx - i * Nth;
+    // four orbitals' loads in flight at a time, summed in orbital order
+    const T *dk = dens + (size_t)inst * n_orb * npt + idx;
     double ru = 0.0;
-    for (int k = 0; k < n_orb; ++k) ru += occ[k] * dens[(size_t)(inst * n_orb + k) * N * Nth + idx];
+    int k = 0;
+    for (; k + 4 <= n_orb; k += 4) {
+        const T d0 = dk[(size_t)k * npt], d1 = dk[(size_t)(k + 1) * npt];
+        const T d2 = dk[(size_t)(k + 2) * npt], d3 = dk[(size_t)(k + 3) * npt];
+        ru += occ[k] * d0;
+        ru += occ[k + 1] * d1;
+        ru += occ[k + 2] * d2;
+        ru += occ[k + 3] * d3;
+    }
+    for (; k < n_orb; ++k) ru += occ[k] * dk[(size_t)k * npt];
     if (rho_frozen) ru += 0.5 * rho_frozen[i] * omega[j];
     const double rho = ru / (2.0 * M_PI * omega[j] * wr[i] * r[i] * r[i]);
     const size_t o = (size_t)inst * N * Nth + idx;
