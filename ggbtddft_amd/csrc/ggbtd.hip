@@ -1165,6 +1165,9 @@ struct GemmOperands {
     int M, N, K;
     const int32_t *active;  // [z] 0: instance idle, skipped (null: all)
 };
+#ifndef TG_KU
+#define TG_KU 9
+#endif
 template <class Epi>
 __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
     __shared__ double red[3][64][4];
@@ -1181,17 +1184,19 @@ __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
     const int kq = (((g.K + 3) / 4 + 3) / 4) * 4;  // quarter of K, multiple of 4
     const int kb = wave * kq, ke = min(g.K, kb + kq);
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = kb; k0 < ke; k0 += 32) {  // 16 loads in flight per lane
-        double av[8], bv[8];
+    // TG_KU loads of each operand in flight per lane; 9 covers a quarter of
+    // K = 129 (N_theta or the multipole count at L_max = 64) in one round
+    for (int k0 = kb; k0 < ke; k0 += 4 * TG_KU) {
+        double av[TG_KU], bv[TG_KU];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < TG_KU; ++u) {
             const int k = k0 + 4 * u + kr;
             const bool ok = k < ke;
             av[u] = (ok && mok) ? Ar[(long)k * g.sak] : 0.0;
             bv[u] = (ok && nok) ? Bc[(long)k * g.sbk] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = mfma64(av[u], bv[u], acc);
+        for (int u = 0; u < TG_KU; ++u) acc = mfma64(av[u], bv[u], acc);
     }
     if (wave > 0)
 #pragma unroll
