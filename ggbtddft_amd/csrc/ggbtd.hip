@@ -1262,6 +1262,10 @@ static __device__ void affine_scan(double &al, double &be, int dir, double (*sm)
 // hp: the ratio powers and 1/r, from the host (shared by all instances):
 // hp[L][a] = (r_{a-1}/r_a)^{L+1} (0 at a = 0), hp[n_har + L][a] = (r_a/r_{a+1})^L
 // (a < N - 1), hp[2 n_har][a] = 1/r_a
+// CS: radial nodes per thread at most (N <= CS * WG); the thread's alpha,
+// gamma, q and 1/r are loaded once into registers (one round of loads) and
+// reused by the four passes, which keep the arithmetic order of the scans.
+template <int CS>
 __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
                                                 const double *__restrict__ hp,
                                                 double *__restrict__ V, int N, int n_har,
@@ -1278,44 +1282,69 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
     const double *ir = hp + (size_t)2 * n_har * N;
     const int cs = (N + WG - 1) / WG;
     const int s0 = min(N, t * cs), s1 = min(N, s0 + cs);
+    const int n = s1 - s0;  // nodes of this thread, <= CS
+    double ra[CS], rg[CS], rq[CS + 1], ri[CS + 1];
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+        const bool in = c < n;
+        ra[c] = in ? alpha[s0 + c] : 0.0;
+        rg[c] = in ? gamma[s0 + c] : 0.0;
+        rq[c] = in ? qL[s0 + c] : 0.0;
+        ri[c] = in ? ir[s0 + c] : 0.0;
+    }
+    // q and 1/r of the node after the chunk (the suffix of its last node)
+    const double qn = (n > 0 && s1 < N) ? qL[s1] : 0.0, irn = (n > 0 && s1 < N) ? ir[s1] : 0.0;
+    rq[CS] = 0.0;
+    ri[CS] = 0.0;
     // ---- prefix: A_a = alpha_a A_{a-1} + beta_a
     double al = 1.0, be = 0.0;
-    for (int a = s0; a < s1; ++a) {
-        al *= alpha[a];
-        be = be * alpha[a] + qL[a] * ir[a];
-    }
+#pragma unroll
+    for (int c = 0; c < CS; ++c)
+        if (c < n) {
+            al *= ra[c];
+            be = be * ra[c] + rq[c] * ri[c];
+        }
     affine_scan(al, be, 1, sm);
     edge[0][t] = be;  // A at the end of this chunk
     // ---- suffix: B_a = gamma_a B_{a+1} + gamma_a q_{a+1}/r_{a+1}, B_{N-1} = 0
     double sa = 1.0, sb = 0.0;
-    for (int a = s1 - 1; a >= s0; --a) {
-        if (a == N - 1) {
-            sa = 0.0;
-            sb = 0.0;
-            continue;
+#pragma unroll
+    for (int c = CS - 1; c >= 0; --c)
+        if (c < n) {
+            if (s0 + c == N - 1) {
+                sa = 0.0;
+                sb = 0.0;
+                continue;
+            }
+            const bool nx = c + 1 < n;
+            const double q1 = nx ? rq[c + 1] : qn, i1 = nx ? ri[c + 1] : irn;
+            const double gam = rg[c];
+            sa *= gam;
+            sb = sb * gam + gam * q1 * i1;
         }
-        const double gam = gamma[a];
-        sa *= gam;
-        sb = sb * gam + gam * qL[a + 1] * ir[a + 1];
-    }
     affine_scan(sa, sb, -1, sm);
     edge[1][t] = sb;  // B at the start of this chunk
     __syncthreads();
     double A = t > 0 ? edge[0][t - 1] : 0.0;
-    for (int a = s0; a < s1; ++a) {
-        A = A * alpha[a] + qL[a] * ir[a];
-        VL[a] = A;
+    double va[CS];
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+        if (c < n) A = A * ra[c] + rq[c] * ri[c];
+        va[c] = A;
     }
     double Bv = t < WG - 1 ? edge[1][t + 1] : 0.0;
-    for (int a = s1 - 1; a >= s0; --a) {
-        if (a == N - 1) {
-            Bv = 0.0;
-        } else {
-            const double gam = gamma[a];
-            Bv = gam * (Bv + qL[a + 1] * ir[a + 1]);
+#pragma unroll
+    for (int c = CS - 1; c >= 0; --c)
+        if (c < n) {
+            if (s0 + c == N - 1) {
+                Bv = 0.0;
+            } else {
+                const bool nx = c + 1 < n;
+                const double q1 = nx ? rq[c + 1] : qn, i1 = nx ? ri[c + 1] : irn;
+                Bv = rg[c] * (Bv + q1 * i1);
+            }
+            VL[s0 + c] = va[c] + Bv;
         }
-        VL[a] += Bv;
-    }
 }
 
 // dV = sum_L V_L(r_i) P_L(chi_j) + v_xc - v_gs and phase = mask e^{-i dV dt/2}:
@@ -2061,7 +2090,9 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
         gq.K = Nth;
         QmomEpi eq{ctx->q, N, nh};
         k_tile_gemm<<<dim3((N + 15) / 16, (nh + 15) / 16, p.n_inst), WG, 0, st>>>(gq, eq);
-        k_hartree<<<dim3(nh, p.n_inst), WG, 0, st>>>(ctx->q, ctx->hpow, ctx->V, N, nh, ctx->active);
+        const int cs = (N + WG - 1) / WG;  // N <= 4096 (ggb_create): cs <= 16
+        auto har = cs <= 2 ? k_hartree<2> : cs <= 4 ? k_hartree<4> : cs <= 8 ? k_hartree<8> : k_hartree<16>;
+        har<<<dim3(nh, p.n_inst), WG, 0, st>>>(ctx->q, ctx->hpow, ctx->V, N, nh, ctx->active);
     }
     GemmOperands gv;  // V(i, theta_j) = sum_L V[L][i] PLT[L][j]
     gv.A = ctx->V;
