@@ -231,7 +231,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 T *__restrict__ Y, int N, int nk_max, int C,
                                                 int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
-    __shared__ double red[3][64][4];
+    __shared__ double red[4][64][4];
     const tile3 tb = xcd_tile();  // (eigen tile, column tile, l): tiles of one l share psi
     const int l = tb.z;
     const int nk = nkeep[l];
@@ -264,32 +264,18 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 #pragma unroll
         for (int u = 0; u < KU; ++u) acc = mfma_t<T>::op(av[u], bv[u], acc);
     }
-    if (wave > 0)
+    // partial tiles through LDS (wave 0's in slot 3); wave w then finishes
+    // accumulator row r = w, so the four epilogues run side by side
 #pragma unroll
-        for (int r = 0; r < 4; ++r) red[wave - 1][lane][r] = (double)acc[r];
+    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : 3][lane][r] = (double)acc[r];
     __syncthreads();
-    if (wave > 0) return;
-    double v4d[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-        v4d[r] = (((double)acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
-    const double2 *kphl = kph + (size_t)l * nk_max;
-    T o[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int a = a0 + mfma_t<T>::row(kr, r);
-        const double v = v4d[r];
-        const double vo = __shfl_xor(v, 1);
-        o[r] = kin_factor<T>(v, vo, p, kphl[a < nk ? a : 0]);
-    }
-    if (colok) {
-        T *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max + a0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int ar = mfma_t<T>::row(kr, r);
-            if (a0 + ar < nk) Yo[ar] = o[r];
-        }
-    }
+    const int r = wave;
+    const double v = ((red[3][lane][r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+    const double vo = __shfl_xor(v, 1);
+    const int ar = mfma_t<T>::row(kr, r);
+    const int a = a0 + ar;
+    const T o = kin_factor<T>(v, vo, p, kph[(size_t)l * nk_max + (a < nk ? a : 0)]);
+    if (colok && a < nk) Y[(((size_t)l * C + c) * 2 + p) * nk_max + a] = o;
 }
 
 // Kinetic propagator, pass 2: psi[l][c][i] = sum_a QT[l][a][i] Y[l][c][a].
@@ -311,7 +297,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ QT,
                                                 double *__restrict__ psi, int N, int nk_max,
                                                 int C, int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
-    __shared__ double red[3][64][4];
+    __shared__ double red[4][64][4];
     const tile3 tb = xcd_tile();  // (radial tile, column tile, l): tiles of one l share Y
     const int l = tb.z;
     const int nk = nkeep[l];
@@ -349,22 +335,20 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ QT,
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc = mfma_t<T>::op(av[u][j], bv[u][j], acc);
     }
-    if (wave > 0)
+    // partial tiles through LDS (wave 0's in slot 3); wave w then finishes
+    // accumulator row r = w, so the four epilogues run side by side
 #pragma unroll
-        for (int r = 0; r < 4; ++r) red[wave - 1][lane][r] = (double)acc[r];
+    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : 3][lane][r] = (double)acc[r];
     __syncthreads();
-    if (wave > 0) return;
     double *Xl = psi + (size_t)l * C * N * 2;
     double rc = 0.0;
     const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, true, &rc) : nullptr;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int i = i0 + mfma_t<T>::row(kr, r);
-        double v = (((double)acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
-        if (colok && i < N) {
-            if (rrow) v += rc * rrow[i];
-            Xl[((size_t)c * N + i) * 2 + p] = v;
-        }
+    const int r = wave;
+    const int i = i0 + mfma_t<T>::row(kr, r);
+    double v = ((red[3][lane][r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+    if (colok && i < N) {
+        if (rrow) v += rc * rrow[i];
+        Xl[((size_t)c * N + i) * 2 + p] = v;
     }
 }
 
@@ -1170,7 +1154,7 @@ struct GemmOperands {
 #endif
 template <class Epi>
 __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
-    __shared__ double red[3][64][4];
+    __shared__ double red[4][64][4];
     const tile3 tb = xcd_tile();  // n tiles of one m tile run on one XCD and share A in its L2
     const int z = tb.z;
     if (g.active && !g.active[z]) return;  // block-uniform
@@ -1198,17 +1182,15 @@ __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
 #pragma unroll
         for (int u = 0; u < TG_KU; ++u) acc = mfma64(av[u], bv[u], acc);
     }
-    if (wave > 0)
+    // partial tiles through LDS (wave 0's in slot 3); wave w then finishes
+    // accumulator row r = w, so the four epilogues run side by side
 #pragma unroll
-        for (int r = 0; r < 4; ++r) red[wave - 1][lane][r] = acc[r];
+    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : 3][lane][r] = acc[r];
     __syncthreads();
-    if (wave > 0) return;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int mo = m0 + mfma_t<double>::row(kr, r);
-        const double v = ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
-        if (mo < g.M && nok) epi(z, mo, n, v);
-    }
+    const int r = wave;
+    const int mo = m0 + mfma_t<double>::row(kr, r);
+    const double v = ((red[3][lane][r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+    if (mo < g.M && nok) epi(z, mo, n, v);
 }
 
 // multipole moments q_L(i) = sum_j P_L(chi_j) rho_u(i, j): M = L, N = i, K = j
