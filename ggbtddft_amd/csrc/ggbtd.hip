@@ -223,15 +223,19 @@ static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, in
 #define KIN_KU 8
 #endif
 #define KU KIN_KU
+#ifndef KIN_NW
+#define KIN_NW 4  // waves per work-group of the narrow passes (K split over them), >= 4
+#endif
+#define KIN_WG (64 * KIN_NW)
 template <typename T>
-__global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
+__global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 const double2 *__restrict__ kph,
                                                 const int32_t *__restrict__ nkeep,
                                                 const double *__restrict__ psi, KinRef ref,
                                                 T *__restrict__ Y, int N, int nk_max, int C,
                                                 int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
-    __shared__ double red[4][64][4];
+    __shared__ double red[KIN_NW][64][4];
     const tile3 tb = xcd_tile();  // (eigen tile, column tile, l): tiles of one l share psi
     const int l = tb.z;
     const int nk = nkeep[l];
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
     const bool colok = c < C;
     const T *Ql = Q + (size_t)l * N * nk_max + a0 + lc;
     const double *Xl = psi + ((size_t)l * C + (colok ? c : 0)) * N * 2 + p;
-    const int kq = (((N + 3) / 4 + 3) / 4) * 4;  // quarter of K, multiple of 4
+    const int kq = (((N + KIN_NW - 1) / KIN_NW + 3) / 4) * 4;  // K share of a wave, multiple of 4
     const int kb = wave * kq, ke = min(N, kb + kq);
     double rc = 0.0;
     const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, false, &rc) : nullptr;
@@ -264,13 +268,15 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 #pragma unroll
         for (int u = 0; u < KU; ++u) acc = mfma_t<T>::op(av[u], bv[u], acc);
     }
-    // partial tiles through LDS (wave 0's in slot 3); wave w then finishes
+    // partial tiles through LDS (wave 0's in the last slot); wave w < 4 then finishes
     // accumulator row r = w, so the four epilogues run side by side
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : 3][lane][r] = (double)acc[r];
+    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : KIN_NW - 1][lane][r] = (double)acc[r];
     __syncthreads();
+    if (wave >= 4) return;
     const int r = wave;
-    const double v = ((red[3][lane][r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+    double v = red[KIN_NW - 1][lane][r];
+    for (int w = 0; w < KIN_NW - 1; ++w) v += red[w][lane][r];
     const double vo = __shfl_xor(v, 1);
     const int ar = mfma_t<T>::row(kr, r);
     const int a = a0 + ar;
@@ -291,13 +297,13 @@ __global__ __launch_bounds__(WG) void k_kin_fwd(const T *__restrict__ Q,
 #endif
 #define KU2 KIN_KU2
 template <typename T>
-__global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ QT,
+__global__ __launch_bounds__(KIN_WG) void k_kin_bwd(const T *__restrict__ QT,
                                                 const int32_t *__restrict__ nkeep,
                                                 const T *__restrict__ Y, KinRef ref,
                                                 double *__restrict__ psi, int N, int nk_max,
                                                 int C, int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
-    __shared__ double red[4][64][4];
+    __shared__ double red[KIN_NW][64][4];
     const tile3 tb = xcd_tile();  // (radial tile, column tile, l): tiles of one l share Y
     const int l = tb.z;
     const int nk = nkeep[l];
@@ -314,7 +320,7 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ QT,
     // 16 consecutive i (one 128-byte run per row, as pass 1 reads Q)
     const T *Qc = QT + (size_t)l * nk_max * N + (im_ < N ? im_ : 0) + (size_t)4 * kr * N;
     const T *Yr = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max + 4 * kr;
-    const int kq = (((kend + 15) / 16 + 3) / 4) * 16;  // quarter of K, multiple of 16
+    const int kq = (((kend + 15) / 16 + KIN_NW - 1) / KIN_NW) * 16;  // K share of a wave, multiple of 16
     const int kb = wave * kq, ke = min(kend, kb + kq);
     v4 acc = {0, 0, 0, 0};
     for (int k0 = kb; k0 < ke; k0 += 16 * KU2) {
@@ -335,17 +341,19 @@ __global__ __launch_bounds__(WG) void k_kin_bwd(const T *__restrict__ QT,
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc = mfma_t<T>::op(av[u][j], bv[u][j], acc);
     }
-    // partial tiles through LDS (wave 0's in slot 3); wave w then finishes
+    // partial tiles through LDS (wave 0's in the last slot); wave w < 4 then finishes
     // accumulator row r = w, so the four epilogues run side by side
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : 3][lane][r] = (double)acc[r];
+    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : KIN_NW - 1][lane][r] = (double)acc[r];
     __syncthreads();
+    if (wave >= 4) return;
     double *Xl = psi + (size_t)l * C * N * 2;
     double rc = 0.0;
     const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, true, &rc) : nullptr;
     const int r = wave;
     const int i = i0 + mfma_t<T>::row(kr, r);
-    double v = ((red[3][lane][r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+    double v = red[KIN_NW - 1][lane][r];
+    for (int w = 0; w < KIN_NW - 1; ++w) v += red[w][lane][r];
     if (colok && i < N) {
         if (rrow) v += rc * rrow[i];
         Xl[((size_t)c * N + i) * 2 + p] = v;
@@ -1182,7 +1190,7 @@ __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
 #pragma unroll
         for (int u = 0; u < TG_KU; ++u) acc = mfma64(av[u], bv[u], acc);
     }
-    // partial tiles through LDS (wave 0's in slot 3); wave w then finishes
+    // partial tiles through LDS (wave 0's in the last slot); wave w < 4 then finishes
     // accumulator row r = w, so the four epilogues run side by side
 #pragma unroll
     for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : 3][lane][r] = acc[r];
@@ -2113,12 +2121,12 @@ static int kin_variant(const ggb_ctx *ctx) {
 // launch of a kinetic product kernel; with events (timing) through
 // hipExtLaunchKernel, which stamps e0/e1 with the kernel's own start and end
 template <typename... KArgs, typename... Args>
-static void kin_launch(void (*k)(KArgs...), dim3 g, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
-                       Args... args) {
+static void kin_launch(void (*k)(KArgs...), dim3 g, int threads, hipStream_t st, hipEvent_t e0,
+                       hipEvent_t e1, Args... args) {
     if (e0)
-        hipExtLaunchKernelGGL(k, g, dim3(WG), 0, st, e0, e1, 0, args...);
+        hipExtLaunchKernelGGL(k, g, dim3(threads), 0, st, e0, e1, 0, args...);
     else
-        k<<<g, WG, 0, st>>>(args...);
+        k<<<g, threads, 0, st>>>(args...);
 }
 
 // e0/e1 (optional): events stamped with the start and end of the pass's
@@ -2153,16 +2161,16 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh, hipEvent_t e0 =
         const float2 *psi32 = ctx->psi32;
         const int N = p.n_r, nkm = p.nk_max, C = ctx->C, no = p.n_orb;
         if (pass == 0 && k32)
-            kin_launch(k_kin_fwd_tiled<T, 32>, gf, ctx->stream, e0, e1, Q,
+            kin_launch(k_kin_fwd_tiled<T, 32>, gf, WG, ctx->stream, e0, e1, Q,
                                   kph, nkeep, psi_in, psi32, ref, Y, N, nkm, C, no);
         else if (pass == 0)
-            kin_launch(k_kin_fwd_tiled<T, 16>, gf, ctx->stream, e0, e1, Q,
+            kin_launch(k_kin_fwd_tiled<T, 16>, gf, WG, ctx->stream, e0, e1, Q,
                                   kph, nkeep, psi_in, psi32, ref, Y, N, nkm, C, no);
         else if (k32)
-            kin_launch(k_kin_bwd_tiled<T, 32>, gb, ctx->stream, e0, e1,
+            kin_launch(k_kin_bwd_tiled<T, 32>, gb, WG, ctx->stream, e0, e1,
                                   (const T *)Q, nkeep, (const T *)Y, ref, psi_out, N, nkm, C, no);
         else
-            kin_launch(k_kin_bwd_tiled<T, 16>, gb, ctx->stream, e0, e1,
+            kin_launch(k_kin_bwd_tiled<T, 16>, gb, WG, ctx->stream, e0, e1,
                                   (const T *)Q, nkeep, (const T *)Y, ref, psi_out, N, nkm, C, no);
         return;
     }
@@ -2172,10 +2180,10 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh, hipEvent_t e0 =
     const int N = p.n_r, nkm = p.nk_max, C = ctx->C, no = p.n_orb;
     if (pass == 0) {
         const double *psi_in = ctx->psi;
-        kin_launch(k_kin_fwd<T>, dim3(p.nk_max / 16, ncol, p.n_l), ctx->stream, e0, e1, Q, kph, nkeep,
+        kin_launch(k_kin_fwd<T>, dim3(p.nk_max / 16, ncol, p.n_l), KIN_WG, ctx->stream, e0, e1, Q, kph, nkeep,
                    psi_in, ref, Y, N, nkm, C, no);
     } else {
-        kin_launch(k_kin_bwd<T>, dim3((p.n_r + 15) / 16, ncol, p.n_l), ctx->stream, e0, e1,
+        kin_launch(k_kin_bwd<T>, dim3((p.n_r + 15) / 16, ncol, p.n_l), KIN_WG, ctx->stream, e0, e1,
                    reinterpret_cast<const T *>(ctx->QT), nkeep, (const T *)Y, ref, ctx->psi, N, nkm,
                    C, no);
     }
