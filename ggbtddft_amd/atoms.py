@@ -22,6 +22,19 @@ _SHELLS = {
     "Xe": (5, 54, ((1, 0), (2, 0), (2, 1), (3, 0), (3, 1), (3, 2), (4, 0), (4, 1), (4, 2),
                    (5, 0), (5, 1))),
 }
+# The reference propagates only these valence orbitals (n, l, m) over a
+# frozen core, in this order (getSimulationStates, internal.py:1162-1202),
+# and its analysis script indexes its per-orbital files by that order
+# (getKSOL/getKSOM, internal.py:16-42).
+REFERENCE_ACTIVE = {
+    "H": ((1, 0, 0),),
+    "He": ((1, 0, 0),),
+    "Ne": ((2, 0, 0), (2, 1, 0), (2, 1, 1)),
+    "Ar": ((3, 0, 0), (3, 1, 0), (3, 1, 1)),
+    "Kr": ((4, 0, 0), (4, 1, 0), (4, 1, 1)),
+    "Xe": ((4, 0, 0), (5, 0, 0), (4, 1, 0), (4, 1, 1), (5, 1, 0), (5, 1, 1), (4, 2, 0), (4, 2, 1),
+           (4, 2, 2)),
+}
 _NAMES = {"hydrogen": "H", "helium": "He", "neon": "Ne", "argon": "Ar", "krypton": "Kr",
           "xenon": "Xe"}
 _L_LETTER = "spdfghik"
@@ -59,3 +72,12 @@ class Atom:
     @property
     def n_electrons(self):
         return sum(o for (_, _, o) in self.shells)
+
+    def reference_order(self):
+        """(order, n_active): the orbital indices with the reference's active
+        orbitals first, in its order, then the remaining (core) orbitals in
+        this propagator's order; n_active = how many are the reference's."""
+        key = [(n, l, m) for (n, l, m, _, _) in self.orbitals]
+        active = [key.index(o) for o in REFERENCE_ACTIVE[self.symbol]]
+        rest = [k for k in range(len(key)) if k not in active]
+        return active + rest, len(active)

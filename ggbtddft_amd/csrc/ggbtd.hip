@@ -226,6 +226,8 @@ static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, in
 #ifndef KIN_NW
 #define KIN_NW 4  // waves per work-group of the narrow passes (K split over them), >= 4
 #endif
+// waves 0..3 finish accumulator rows 0..3 of every tile after the K split
+static_assert(KIN_NW >= 4 && KIN_NW <= 16, "KIN_NW: 4 to 16 waves per narrow work-group");
 #define KIN_WG (64 * KIN_NW)
 template <typename T>
 __global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
@@ -1061,6 +1063,46 @@ static __device__ void obs_reduce_pair(const ObsArgs &o, int idx) {
 }
 __global__ __launch_bounds__(64) void k_obs_reduce(ObsArgs o) { obs_reduce_pair(o, blockIdx.x); }
 
+// Per-(column, l) channel dipole of the state after the step,
+//   chan[row][c][l] = <x_l | r (c(l, m) x_{l+1} + c(l-1, m) x_{l-1})>
+// (conjugate on the left; dipoleMoment.cuh:1-47 reduced as by cublasZdotc,
+// :71), one wave per (l, column), fixed-order reduction.  Rows l < m and idle
+// instances hold zero.
+struct ChanArgs {
+    const double *psi, *r;
+    const int32_t *mcol, *active;
+    double2 *chan;               // [rows][C][Nl]
+    int N, Nl, C, n_orb;
+    const int *cur, *origin;     // graph replay: row = cur[0] - origin[0] + local
+    int local;
+};
+__global__ __launch_bounds__(64) void k_chan_dip(ChanArgs a) {
+    const int l = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
+    const int m = a.mcol[c % a.n_orb];
+    const size_t row = a.cur ? (size_t)(a.cur[0] - a.origin[0] + a.local) : (size_t)a.local;
+    double sr = 0.0, si = 0.0;
+    if (l >= m && !(a.active && !a.active[c / a.n_orb])) {
+        const size_t plane = (size_t)a.C * a.N;  // one l of the state, complex values
+        const double2 *x = reinterpret_cast<const double2 *>(a.psi) + (size_t)l * plane + (size_t)c * a.N;
+        const bool up = l + 1 < a.Nl, dn = l - 1 >= m;
+        const double cu = up ? c_lm(l, m) : 0.0, cd = dn ? c_lm(l - 1, m) : 0.0;
+        const double2 *xu = up ? x + plane : x;
+        const double2 *xd = dn ? x - plane : x;
+        for (int i = lane; i < a.N; i += 64) {
+            const double2 v = x[i], u = xu[i], d = xd[i];
+            const double ri = a.r[i];
+            const double zr = ri * (cu * u.x + cd * d.x), zi = ri * (cu * u.y + cd * d.y);
+            sr += v.x * zr + v.y * zi;
+            si += v.x * zi - v.y * zr;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_down(sr, off);
+        si += __shfl_down(si, off);
+    }
+    if (lane == 0) a.chan[(row * a.C + c) * a.Nl + l] = make_double2(sr, si);
+}
+
 // advances the replayed graph's step counter
 __global__ void k_bump(int *cur, int by) {
     if (threadIdx.x == 0 && blockIdx.x == 0) cur[0] += by;
@@ -1605,6 +1647,11 @@ struct ggb_ctx {
     int32_t *l0;
     double4 *ab;
     size_t obs_cap;
+    // optional per-(column, l) channel dipoles of every step (ggb_set_channels)
+    bool channels;
+    double2 *chan;   // [rows][C][n_l]
+    size_t chan_cap; // complex values
+    int chan_rows;   // rows written by the last ggb_run
     // optional hipGraph of graph_steps steps (ggb_set_graph)
     int graph_steps;
     hipGraphExec_t graph_exec;
@@ -1662,7 +1709,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
                     ctx->r, ctx->hpow, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
-                    ctx->l0, ctx->ab, ctx->psi32, ctx->active};
+                    ctx->l0, ctx->ab, ctx->psi32, ctx->active, ctx->chan};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -1732,6 +1779,10 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->n_field = 0;
     ctx->field_ring = false;
     ctx->obs_cap = 0;
+    ctx->channels = false;
+    ctx->chan = nullptr;
+    ctx->chan_cap = 0;
+    ctx->chan_rows = 0;
     ctx->kph = nullptr;
     ctx->psi = ctx->Y = ctx->Q = ctx->QT = ctx->B = ctx->PL = ctx->PLT = ctx->chi = ctx->omega = nullptr;
     ctx->r = ctx->hpow = ctx->wr = ctx->v_gs = ctx->mask = ctx->occ = ctx->rho_frozen = nullptr;
@@ -2206,11 +2257,31 @@ static int launch_kinetic(ggb_ctx *ctx, bool fresh = false) {
 
 static hipError_t no_mark() { return hipSuccess; }
 
+// channel dipoles of the state after the step into dst (row 0, or the
+// replay row cur[0] - cur[1] + step)
+static void launch_channels(ggb_ctx *ctx, int step, double2 *dst, const int *cur) {
+    ChanArgs a;
+    a.psi = ctx->psi;
+    a.r = ctx->r;
+    a.mcol = ctx->mcol;
+    a.active = ctx->active;
+    a.chan = dst;
+    a.N = ctx->p.n_r;
+    a.Nl = ctx->p.n_l;
+    a.C = ctx->C;
+    a.n_orb = ctx->p.n_orb;
+    a.cur = cur;
+    a.origin = cur ? cur + 1 : nullptr;
+    a.local = cur ? step : 0;
+    k_chan_dip<<<dim3(ctx->p.n_l, ctx->C), 64, 0, ctx->stream>>>(a);
+}
+
 template <typename Mark = hipError_t (*)()>
 static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int *cur = nullptr,
-                              Mark between = no_mark) {
+                              Mark between = no_mark, double2 *chan_dst = nullptr) {
     launch_angular<1>(ctx, step, cur);
     HIPCHK(ctx, hipGetLastError());
+    if (chan_dst) launch_channels(ctx, step, chan_dst, cur);
     HIPCHK(ctx, between());  // timing event between the angular half and the rebuild
     ObsArgs ob;
     ob.part = ctx->obs_part;
@@ -2362,7 +2433,9 @@ static int run_graph(ggb_ctx *ctx, int step0, int nsteps, int *done) {
         for (int s = 0; s < G && !rc; ++s) {
             rc = launch_first_half(ctx, s, ctx->dcur);
             if (!rc) rc = launch_kinetic(ctx, true);
-            if (!rc) rc = launch_second_half(ctx, s, ctx->obs, ctx->dcur);
+            if (!rc)
+                rc = launch_second_half(ctx, s, ctx->obs, ctx->dcur, no_mark,
+                                        ctx->channels ? ctx->chan : nullptr);
         }
         if (!rc) k_bump<<<1, 64, 0, ctx->stream>>>(ctx->dcur, G);
         hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
@@ -2411,6 +2484,17 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         HIPCHK(ctx, dalloc(&ctx->obs, need));
         ctx->obs_cap = need;
     }
+    const size_t cneed = ctx->channels ? (size_t)nsteps * ctx->C * ctx->p.n_l : 0;
+    if (cneed > ctx->chan_cap) {
+        drop_graph(ctx);
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->chan) (void)hipFree(ctx->chan);
+        ctx->chan = nullptr;
+        ctx->chan_cap = 0;
+        HIPCHK(ctx, dalloc(&ctx->chan, cneed));
+        ctx->chan_cap = cneed;
+    }
+    ctx->chan_rows = ctx->channels ? nsteps : 0;
     const int ne = GGB_NPHASE + 1;
     if (ctx->timing) {
         const size_t want = (size_t)nsteps * ne;
@@ -2444,8 +2528,10 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         HIPCHK(ctx, mark(s, 2));
         if (!rc) rc = launch_kinetic_pass(ctx, 1, true, kev(s, 3), kev(s, 4));
         HIPCHK(ctx, mark(s, 3));
-        if (!rc) rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3, nullptr,
-                                         [&]() { return mark(s, 4); });
+        if (!rc)
+            rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3, nullptr,
+                                    [&]() { return mark(s, 4); },
+                                    ctx->channels ? ctx->chan + (size_t)s * ctx->C * ctx->p.n_l : nullptr);
         HIPCHK(ctx, mark(s, 5));
         if (rc) return rc;
     }
@@ -2479,6 +2565,30 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         }
         delete[] h;
     }
+    return GGB_OK;
+}
+
+extern "C" int ggb_set_channels(ggb_ctx *ctx, int32_t on) {
+    if (!ctx) return GGB_E_ARG;
+    if (on != 0 && on != 1) return fail(ctx, GGB_E_ARG, "ggb_set_channels: on must be 0 or 1");
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->channels != (on == 1)) drop_graph(ctx);  // the captured step changes
+    ctx->channels = on == 1;
+    ctx->chan_rows = 0;
+    return GGB_OK;
+}
+
+extern "C" int ggb_get_channels(ggb_ctx *ctx, int32_t nsteps, double *chan) {
+    if (!ctx) return GGB_E_ARG;
+    if (!chan) return fail(ctx, GGB_E_ARG, "ggb_get_channels: null buffer");
+    if (nsteps != ctx->chan_rows || nsteps < 1)
+        return fail(ctx, GGB_E_STATE, "ggb_get_channels: the last ggb_run did not record that many steps");
+    HIPCHK(ctx, hipSetDevice(ctx->p.device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(chan, ctx->chan, (size_t)nsteps * ctx->C * ctx->p.n_l * sizeof(double2),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return GGB_OK;
 }
 

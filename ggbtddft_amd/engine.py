@@ -136,9 +136,14 @@ class Propagator:
         assert m.shape == (self.n_inst,)
         self._check(self.lib.ggb_set_active(self.ctx, _lib.iptr(m)), "ggb_set_active")
 
-    def run(self, step0, nsteps, observables=True):
+    def run(self, step0, nsteps, observables=True, channels=False):
         """Propagate nsteps steps; returns dict(dip, acc, norm) of shape
-        [nsteps, n_inst, n_orb] (None values when observables=False)."""
+        [nsteps, n_inst, n_orb] (None when observables=False) and, with
+        channels, chan [nsteps, n_inst, n_orb, n_l] complex: the per-(orbital,
+        l) channel dipoles of the reference's output files."""
+        if channels != getattr(self, "_channels", False):
+            self._check(self.lib.ggb_set_channels(self.ctx, 1 if channels else 0), "ggb_set_channels")
+            self._channels = bool(channels)
         if not observables:
             self._check(self.lib.ggb_run(self.ctx, step0, nsteps, None, None, None), "ggb_run")
             return None
@@ -148,7 +153,13 @@ class Propagator:
         self._check(self.lib.ggb_run(self.ctx, step0, nsteps, _lib.dptr(dip), _lib.dptr(acc),
                                      _lib.dptr(nrm)), "ggb_run")
         sh = (nsteps, self.n_inst, self.sys.n_orb)
-        return dict(dip=dip.reshape(sh), acc=acc.reshape(sh), norm=nrm.reshape(sh))
+        out = dict(dip=dip.reshape(sh), acc=acc.reshape(sh), norm=nrm.reshape(sh))
+        if channels and nsteps > 0:
+            ch = np.empty((nsteps, self.C, self.sys.n_l), dtype=np.complex128)
+            self._check(self.lib.ggb_get_channels(self.ctx, nsteps, _lib.dptr(ch.view(np.float64))),
+                        "ggb_get_channels")
+            out["chan"] = ch.reshape(sh + (self.sys.n_l,))
+        return out
 
     def set_kinetic_variant(self, variant):
         """Kinetic kernel variant: index or name in _lib.KIN_VARIANTS."""

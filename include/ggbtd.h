@@ -133,8 +133,10 @@ int ggb_upload_columns(ggb_ctx *ctx, int32_t c0, int32_t nc, const double *psi);
 
 /* Mark which instances are in use (active[n_inst], nonzero = in use; all are
  * at creation).  The angular, density and potential kernels skip idle
- * instances (their columns keep stale values; the kinetic products still run
- * over every column), so a serving lane with free slots does less work.  A
+ * instances (their columns keep stale values), and the kinetic products skip
+ * every tile of orbital columns that all belong to idle instances (8
+ * columns in the narrow kernels, 64 in the tiled ones), so a serving lane with free
+ * slots does less work.  A
  * slot being seated must be marked in use before ggb_rebuild_potential.  No
  * counterpart in the reference, which runs its scans one after another. */
 int ggb_set_active(ggb_ctx *ctx, const int32_t *active);
@@ -151,6 +153,21 @@ int ggb_rebuild_potential(ggb_ctx *ctx);
  * observables of dipoleMoment.cuh / dipoleAcceleration.cuh. */
 int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip, double *acc,
             double *norm);
+
+/* Record, in later ggb_run calls, the per-(orbital column, l) channel
+ * dipole of the state after each step,
+ *     chan[c][l] = <x_l | r (c(l, m) x_{l+1} + c(l-1, m) x_{l-1})>
+ * (complex, conjugate on the left, c(l, m) = <l m|cos theta|l+1 m>; zero for
+ * l < m and for idle instances), on = 1, or stop (0, the default).  Replaces
+ * h_calculateDipoleMoment (dipoleMoment.cuh:49-75), whose per-channel values
+ * h_writeDipoleMoment writes one file per (orbital, l)
+ * (h_writeDipoleMoment.cuh:12-22); sum_l Re chan[c][l] is the dip of ggb_run. */
+int ggb_set_channels(ggb_ctx *ctx, int32_t on);
+
+/* Copy the channel dipoles of the last ggb_run (which must have run nsteps
+ * steps with channels on) into chan[nsteps][n_inst*n_orb][n_l] complex
+ * (re, im interleaved). */
+int ggb_get_channels(ggb_ctx *ctx, int32_t nsteps, double *chan);
 
 /* Copy the current orbitals out (same layout as ggb_upload_orbitals). */
 int ggb_download_orbitals(ggb_ctx *ctx, double *psi);

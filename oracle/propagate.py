@@ -130,6 +130,24 @@ class System:
         nrm = P2.sum(axis=(1, 2))
         return dip, acc, nrm
 
+    def channel_dipoles(self, psi):
+        """chan[k, l] = <x_l | r (c_u(l) x_{l+1} + c_d(l) x_{l-1})> of orbital k
+        (complex, conjugate on the left), the per-channel dipole of
+        ``dipoleMoment.cuh:1-47`` reduced by ``cublasZdotc`` (:71);
+        c_u(l) = c(l, m), c_d(l) = c(l - 1, m) (angularCoefficients.cuh d_cu,
+        d_cd), zero for l < m.  sum_l Re chan[k, l] = dip_k."""
+        out = np.zeros((self.norb, self.nl), dtype=np.complex128)
+        for k in range(self.norb):
+            mm = self.m[k]
+            for l in range(mm, self.nl):
+                z = np.zeros(self.N, dtype=np.complex128)
+                if l + 1 < self.nl:
+                    z += angular.c_lm(l, mm) * psi[l + 1, k]
+                if l - 1 >= mm:
+                    z += angular.c_lm(l - 1, mm) * psi[l - 1, k]
+                out[k, l] = np.vdot(psi[l, k], self.r * z)
+        return out
+
     # ---- sub-steps ---------------------------------------------------------
     def v_half(self, psi, dV):
         ph = np.exp(-0.5j * self.dt * dV) * self.mask[:, None]
@@ -166,8 +184,9 @@ class System:
         self.laser(psi, E, 0)
         self.v_half(psi, dV)
 
-    def run(self, E_mid, psi=None, nsteps=None):
-        """Propagate; returns dict(dip, acc, norm [nsteps, norb], psi)."""
+    def run(self, E_mid, psi=None, nsteps=None, channels=False):
+        """Propagate; returns dict(dip, acc, norm [nsteps, norb], psi) and,
+        with channels, chan [nsteps, norb, nl] (channel_dipoles after each step)."""
         psi = self.psi0.copy() if psi is None else psi.copy()
         nsteps = len(E_mid) if nsteps is None else nsteps
         Psi = self.theta_state(psi)
@@ -175,12 +194,18 @@ class System:
         dip = np.empty((nsteps, self.norb))
         acc = np.empty((nsteps, self.norb))
         nrm = np.empty((nsteps, self.norb))
+        chan = np.empty((nsteps, self.norb, self.nl), dtype=np.complex128) if channels else None
         for n in range(nsteps):
             self.step(psi, dV, E_mid[n])
             Psi = self.theta_state(psi)
             dip[n], acc[n], nrm[n] = self.observables(Psi)
+            if channels:
+                chan[n] = self.channel_dipoles(psi)
             dV = self.potential(Psi)
-        return dict(dip=dip, acc=acc, norm=nrm, psi=psi)
+        out = dict(dip=dip, acc=acc, norm=nrm, psi=psi)
+        if channels:
+            out["chan"] = chan
+        return out
 
     def total_acceleration(self, acc, norm, E_end):
         """a(t) = -Z sum_k occ_k <cos/r^2>_k - E(t) sum_k occ_k |psi_k|^2

@@ -31,9 +31,11 @@ class OraclePropagator:
         self.psi[inst] = (self.O.psi0 if psi is None else psi).copy()
         self.dV[inst] = self.O.potential(self.O.theta_state(self.psi[inst]))
 
-    def run(self, step0, nsteps):
+    def run(self, step0, nsteps, channels=False):
         no = self.O.norb
         out = {k: np.zeros((nsteps, self.n_inst, no)) for k in ("dip", "acc", "norm")}
+        if channels:
+            out["chan"] = np.zeros((nsteps, self.n_inst, no, self.O.nl), dtype=np.complex128)
         cap = self.E.shape[1]
         for s in range(nsteps):
             for i in range(self.n_inst):
@@ -41,6 +43,8 @@ class OraclePropagator:
                 Psi = self.O.theta_state(self.psi[i])
                 d, a, n = self.O.observables(Psi)
                 out["dip"][s, i], out["acc"][s, i], out["norm"][s, i] = d, a, n
+                if channels:
+                    out["chan"][s, i] = self.O.channel_dipoles(self.psi[i])
                 self.dV[i] = self.O.potential(Psi)
         return out
 

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from ggbtddft_amd import run as R
-from ggbtddft_amd import spectrum as spec
+from oracle import spectrum as OS
 from ggbtddft_amd.config import RunConfig, read_init
 from oracle import propagate as OP
 
@@ -32,24 +32,32 @@ N_scan = 3
 
 
 def test_driver_matches_oracle(tmp_path):
-    print("case: driver He N=32 Lmax=5, 3 in-situ scans, 400 steps", flush=True)
+    print("case: driver He N=32 Lmax=5, 3 in-situ scans + unperturbed, 400 steps", flush=True)
     cfg = RunConfig(read_init(INIT))
-    base, scans = R.run(cfg, str(tmp_path / "output"), str(tmp_path / "workingData"), max_batch=2)
+    base, scans = R.run(cfg, str(tmp_path / "output"), str(tmp_path / "workingData"), max_batch=2,
+                        sim_date=20261203)
+    assert len(scans) == 4
     O = OP.System("He", cfg.N, cfg.r_max, cfg.lmax, cfg.dt)
     t = cfg.time_grid()
-    for s, delay in enumerate(cfg.delays()):
-        ref = O.run(cfg.supergaussian_field(t - 0.5 * cfg.dt, delay))
+    for s, delay in enumerate(list(cfg.delays()) + [None]):
+        ref = O.run(cfg.supergaussian_field(t - 0.5 * cfg.dt, delay), channels=True)
         e_end = cfg.supergaussian_field(t, delay)
         a_ref = -O.Z * (ref["acc"] @ O.occ) - e_end * (ref["norm"] @ O.occ)
         rel = np.linalg.norm(scans[s]["acc_total"] - a_ref) / np.linalg.norm(a_ref)
         print("scan", s, "acc rel L2", rel, flush=True)
         assert rel < 2e-8
         assert np.abs(scans[s]["norm"] - ref["norm"]).max() < 1e-9
-        _, S = spec.acceleration_spectrum(scans[s]["acc_total"], cfg.dt)
+        ch = scans[s]["chan"][:, 0]
+        assert np.abs(ch - ref["chan"][:, 0]).max() < 2e-8 * np.abs(ref["chan"]).max()
+        # the device spectrum of this signal against the oracle's definition
+        _, S = OS.acceleration_spectrum(scans[s]["acc_total"], cfg.dt)
         assert np.abs(scans[s]["spectrum"] - S).max() <= 1e-11 * S.max()
     acc = np.fromfile(os.path.join(base, "timeDependent", "observables", "dipoleAcceleration",
                                    "scan002", "acc.bin"))
     assert np.array_equal(acc, scans[2]["acc_total"])
+    d = np.fromfile(os.path.join(base, "timeDependent", "observables", "dipoleMoment", "scan003",
+                                 "kso000", "dipL001.bin"), dtype=np.complex128)
+    assert np.array_equal(d, scans[3]["chan"][:, 0, 1])
 
 
 def test_driver_fp32_mixed_close_to_fp64(tmp_path):

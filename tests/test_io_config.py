@@ -49,9 +49,13 @@ def test_working_data_round_trip(tmp_path):
     S = AtomSystem("Ar", 48, 40.0, 6, 0.05)
     io.write_working_data(str(tmp_path), S, np.arange(10) * 0.05)
     psi = io.read_eigenstates(str(tmp_path), S.atom.target_id, S.n_l, S.n_orb, S.N)
-    assert np.array_equal(psi, S.psi0)
+    order, n_active = S.atom.reference_order()
+    assert np.array_equal(psi, S.psi0[:, order])
+    # the reference's active orbitals (3s, 3p0, 3p1) first, then the core
     occ = np.fromfile(os.path.join(tmp_path, "stateParameters", "occ_active.bin"), dtype=np.int32)
-    assert list(occ) == [2, 2, 2, 2, 2, 4, 4]
+    assert list(occ) == [2, 2, 4, 2, 2, 2, 4] and n_active == 3
+    l = np.fromfile(os.path.join(tmp_path, "stateParameters", "l_active.bin"), dtype=np.int32)
+    assert list(l[:3]) == [0, 1, 1]
     r = np.fromfile(os.path.join(tmp_path, "mesh", "r.bin"))
     assert np.array_equal(r, S.grid.r)
 
@@ -59,12 +63,18 @@ def test_working_data_round_trip(tmp_path):
 def test_output_tree(tmp_path):
     S = AtomSystem("He", 32, 30.0, 4, 0.05)
     nt = 20
-    scans = [dict(E=np.ones(nt), dip=np.arange(nt, dtype=float)[:, None], acc_total=np.zeros(nt),
-                  omega=np.zeros(3), spectrum=np.ones(3))]
-    base = io.write_output(str(tmp_path), 7, S, np.arange(nt) * 0.05, scans)
+    chan = np.zeros((nt, 1, S.n_l), dtype=np.complex128)
+    chan[:, 0, 1] = np.arange(nt) + 0.5j
+    scans = [dict(E=np.ones(nt), chan=chan, acc_total=np.zeros(nt), omega=np.zeros(3),
+                  spectrum=np.ones(3))]
+    base = io.write_output(str(tmp_path), 7, S, np.arange(nt) * 0.05, scans, sim_date=20261203)
+    assert base == os.path.join(str(tmp_path), "20261203", "00007")
     d = np.fromfile(os.path.join(base, "timeDependent", "observables", "dipoleMoment", "scan000",
-                                 "kso000", "dipL000.bin"), dtype=np.complex128)
-    assert np.array_equal(d.real, np.arange(nt))
+                                 "kso000", "dipL001.bin"), dtype=np.complex128)
+    assert np.array_equal(d, chan[:, 0, 1])
     z = np.fromfile(os.path.join(base, "timeDependent", "observables", "dipoleMoment", "scan000",
                                  "kso000", "dipL003.bin"), dtype=np.complex128)
     assert z.size == nt and not z.any()
+    assert os.path.exists(os.path.join(base, "params.cuh"))
+    with pytest.raises(ValueError):
+        io.write_output(str(tmp_path), 8, S, np.arange(nt) * 0.05, [dict(scans[0], chan=chan[:, :, :2])])
