@@ -172,36 +172,12 @@ def serving_run(sysm_ar, args):
     return stats
 
 
-def scan_requests(n, seed=0):
-    """The batched scan: n instances from `seed` over He and Ne (uniform),
-    intensity log-uniform in [5e13, 5e14] W/cm^2, wavelength uniform in
-    [600, 2000] nm, sin^2 length uniform integer in [4, 16] cycles (uneven step
-    counts); the grid from the quiver radius alpha = E0 / omega0^2: N = 256,
-    L_max = 32, r_max = 100 below 25 a.u., N = 384, L_max = 48, r_max = 150
-    below 60 a.u., else N = 512, L_max = 64, r_max = 200; dt = 0.05."""
-    from ggbtddft_amd.pulse import Pulse
-    from ggbtddft_amd.serving import Request
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for k in range(n):
-        atom = "He" if rng.uniform() < 0.5 else "Ne"
-        I = float(np.exp(rng.uniform(np.log(5e13), np.log(5e14))))
-        lam = float(rng.uniform(600.0, 2000.0))
-        cyc = int(rng.integers(4, 17))
-        p = Pulse(I, lam, cyc)
-        alpha = p.E0 / p.omega0 ** 2
-        nr, lmax, rmax = ((256, 32, 100.0) if alpha < 25.0 else
-                          (384, 48, 150.0) if alpha < 60.0 else (512, 64, 200.0))
-        reqs.append(Request(rid=k, atom=atom, intensity=I, wavelength=lam, n_cycles=cyc, N=nr,
-                            r_max=rmax, lmax=lmax, dt=0.05))
-    return reqs
-
-
 def scan_run(n):
     """The batched scan of n instances served as one batch (all requests at
     t = 0, one lane per grid key, each lane with a slot per request of its
     key, idle slots skipped once their request is done), FP64 then FP32 mixed;
     orbital-timesteps/s over the whole scan and the FP32 error per instance."""
+    from ggbtddft_amd.scan import scan_requests
     from ggbtddft_amd.serving import Server
     reqs = scan_requests(n)
     keys = {}
