@@ -21,7 +21,8 @@ EXPORTS = ("ggb_abi_version", "ggb_create", "ggb_upload_static", "ggb_upload_orb
            "ggb_field_ring", "ggb_set_field_slice", "ggb_upload_columns", "ggb_set_active",
            "ggb_rebuild_potential", "ggb_footprint", "ggb_selftest_copy", "ggb_set_graph",
            "ggb_spectrum", "ggb_set_kinetic_variant", "ggb_autotune", "ggb_ground_state",
-           "ggb_set_channels", "ggb_get_channels", "ggb_last_error", "ggb_destroy")
+           "ggb_set_channels", "ggb_get_channels", "ggb_spectrum_noalloc", "ggb_last_error",
+           "ggb_destroy")
 KIN_VARIANTS = ("default", "narrow", "tiled16", "tiled32")
 NPHASE = 5
 PHASES = ("half_potential_laser", "kinetic_qt", "kinetic_q", "laser_half_potential_obs",
@@ -117,6 +118,8 @@ def load():
     lib.ggb_spectrum.argtypes = [ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_double,
                                  ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
                                  _dp, ctypes.c_int32, _dp, _dp]
+    lib.ggb_spectrum_noalloc.restype = ctypes.c_int
+    lib.ggb_spectrum_noalloc.argtypes = lib.ggb_spectrum.argtypes
     lib.ggb_ground_state.restype = ctypes.c_int
     lib.ggb_ground_state.argtypes = [ctypes.POINTER(GsProblem), _ip, _dp, _dp, _dp, _dp, _dp, _dp,
                                      _dp, _dp, _dp, _dp, _ip, _ip]

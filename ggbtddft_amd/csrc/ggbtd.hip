@@ -39,6 +39,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2691,76 +2692,182 @@ static hipError_t fft_pow2(double2 *buf, double2 *tmp, long L, double sign, hipS
     return e;
 }
 
+// device buffers of one spectrum: the signal and its windowed copy (n), the
+// power spectrum (M), the three Bluestein/FFT buffers (L) and the harmonic
+// orders and intensities (n_orders)
+struct SpecBufs {
+    double *sig, *x, *pow, *ord, *inten;
+    double2 *A, *B, *T;
+};
+
+static bool spec_args_ok(const double *signal, int32_t n, double dt, double beta, int32_t n_pad,
+                         double omega0, const double *orders, int32_t n_orders,
+                         const double *intensity, const char *who) {
+    if (!signal || n < 16 || n_pad < 0 || !(dt > 0.0) || !(beta >= 0.0) || n_orders < 0 ||
+        (n_orders > 0 && (!orders || !intensity || !(omega0 > 0.0)))) {
+        g_create_err = std::string(who) + ": bad arguments";
+        return false;
+    }
+    if ((2L * n_pad + 1) * n > (1L << 27)) {
+        g_create_err = std::string(who) + ": (2 n_pad + 1) n exceeds 2^27";
+        return false;
+    }
+    return true;
+}
+
+static long spec_fft_length(long M) {
+    long L = 1;
+    while (L < 2 * M - 1) L *= 2;
+    return L;
+}
+
+// the spectrum on stream st into the buffers b; copies in and out, then
+// waits for st.  Returns GGB_OK or GGB_E_HIP (message in g_create_err).
+static int spectrum_on(const SpecBufs &b, hipStream_t st, const double *signal, int32_t n, double dt,
+                       int32_t differentiate, double beta, int32_t n_pad, double omega0,
+                       const double *orders, int32_t n_orders, double *power, double *intensity,
+                       const char *who) {
+    const long M = (2L * n_pad + 1) * n, L = spec_fft_length(M);
+    int rc = GGB_OK;
+    auto hip = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == GGB_OK) {
+            g_create_err = std::string(who) + ": " + what + ": " + hipGetErrorString(e);
+            rc = GGB_E_HIP;
+        }
+        return rc == GGB_OK;
+    };
+    if (!hip(hipMemcpyAsync(b.sig, signal, (size_t)n * 8, hipMemcpyHostToDevice, st), "upload"))
+        return rc;
+    const unsigned gn = (unsigned)((n + WG - 1) / WG);
+    const unsigned gL = (unsigned)((L + WG - 1) / WG);
+    const unsigned gM = (unsigned)((M + WG - 1) / WG);
+    k_spec_window<<<gn, WG, 0, st>>>(b.sig, b.x, n, beta);
+    double *x = b.x;
+    if (differentiate) {
+        k_spec_fd8<<<gn, WG, 0, st>>>(b.x, b.sig, n, 1.0 / (dt * dt));
+        x = b.sig;
+    }
+    k_blue_a<<<gL, WG, 0, st>>>(x, b.A, n, n_pad, M, L);
+    k_blue_b<<<gL, WG, 0, st>>>(b.B, M, L);
+    if (hip(hipGetLastError(), "launch") && hip(fft_pow2(b.A, b.T, L, -1.0, st), "fft") &&
+        hip(fft_pow2(b.B, b.T, L, -1.0, st), "fft")) {
+        k_cmul<<<gL, WG, 0, st>>>(b.A, b.B, L);
+        if (hip(fft_pow2(b.A, b.T, L, 1.0, st), "inverse fft")) {
+            k_blue_power<<<gM, WG, 0, st>>>(b.A, b.pow, M, L);
+            if (n_orders > 0 &&
+                hip(hipMemcpyAsync(b.ord, orders, (size_t)n_orders * 8, hipMemcpyHostToDevice, st), "upload"))
+                k_harmonics<<<n_orders, WG, 0, st>>>(b.pow, b.ord, b.inten, M, dt, omega0);
+            if (hip(hipGetLastError(), "launch") && power)
+                hip(hipMemcpyAsync(power, b.pow, (size_t)M * 8, hipMemcpyDeviceToHost, st), "download");
+            if (rc == GGB_OK && n_orders > 0)
+                hip(hipMemcpyAsync(intensity, b.inten, (size_t)n_orders * 8, hipMemcpyDeviceToHost, st),
+                    "download");
+        }
+    }
+    hip(hipStreamSynchronize(st), "synchronize");
+    return rc;
+}
+
 extern "C" int ggb_spectrum(int32_t device, const double *signal, int32_t n, double dt,
                             int32_t differentiate, double beta, int32_t n_pad, double omega0,
                             const double *orders, int32_t n_orders, double *power, double *intensity) {
-    if (!signal || n < 16 || n_pad < 0 || !(dt > 0.0) || !(beta >= 0.0) || n_orders < 0 ||
-        (n_orders > 0 && (!orders || !intensity || !(omega0 > 0.0)))) {
-        g_create_err = "ggb_spectrum: bad arguments";
+    if (!spec_args_ok(signal, n, dt, beta, n_pad, omega0, orders, n_orders, intensity, "ggb_spectrum"))
         return GGB_E_ARG;
-    }
-    const long M = (2L * n_pad + 1) * n;
-    if (M > (1L << 27)) {
-        g_create_err = "ggb_spectrum: (2 n_pad + 1) n exceeds 2^27";
-        return GGB_E_ARG;
-    }
-    long L = 1;
-    while (L < 2 * M - 1) L *= 2;
+    const long M = (2L * n_pad + 1) * n, L = spec_fft_length(M);
     if (hipSetDevice(device) != hipSuccess) {
         g_create_err = "ggb_spectrum: hipSetDevice failed";
         return GGB_E_HIP;
     }
     hipStream_t st = nullptr;
-    double *dsig = nullptr, *dx = nullptr, *dpow = nullptr, *dord = nullptr, *dint = nullptr;
-    double2 *A = nullptr, *B = nullptr, *T = nullptr;
-    int rc = GGB_OK;
-    auto hip = [&](hipError_t e, const char *what) {
-        if (e != hipSuccess && rc == GGB_OK) {
-            g_create_err = std::string("ggb_spectrum: ") + what + ": " + hipGetErrorString(e);
-            rc = GGB_E_HIP;
-        }
-        return rc == GGB_OK;
-    };
-    if (hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream") &&
-        hip(dalloc(&dsig, (size_t)n), "alloc") && hip(dalloc(&dx, (size_t)n), "alloc") &&
-        hip(dalloc(&dpow, (size_t)M), "alloc") && hip(dalloc(&A, (size_t)L), "alloc") &&
-        hip(dalloc(&B, (size_t)L), "alloc") && hip(dalloc(&T, (size_t)L), "alloc") &&
-        hip(dalloc(&dord, (size_t)(n_orders > 0 ? n_orders : 1)), "alloc") &&
-        hip(dalloc(&dint, (size_t)(n_orders > 0 ? n_orders : 1)), "alloc") &&
-        hip(hipMemcpyAsync(dsig, signal, (size_t)n * 8, hipMemcpyHostToDevice, st), "upload")) {
-        const unsigned gn = (unsigned)((n + WG - 1) / WG);
-        const unsigned gL = (unsigned)((L + WG - 1) / WG);
-        const unsigned gM = (unsigned)((M + WG - 1) / WG);
-        k_spec_window<<<gn, WG, 0, st>>>(dsig, dx, n, beta);
-        double *x = dx;
-        if (differentiate) {
-            k_spec_fd8<<<gn, WG, 0, st>>>(dx, dsig, n, 1.0 / (dt * dt));
-            x = dsig;
-        }
-        k_blue_a<<<gL, WG, 0, st>>>(x, A, n, n_pad, M, L);
-        k_blue_b<<<gL, WG, 0, st>>>(B, M, L);
-        if (hip(hipGetLastError(), "launch") && hip(fft_pow2(A, T, L, -1.0, st), "fft") &&
-            hip(fft_pow2(B, T, L, -1.0, st), "fft")) {
-            k_cmul<<<gL, WG, 0, st>>>(A, B, L);
-            if (hip(fft_pow2(A, T, L, 1.0, st), "inverse fft")) {
-                k_blue_power<<<gM, WG, 0, st>>>(A, dpow, M, L);
-                if (n_orders > 0 &&
-                    hip(hipMemcpyAsync(dord, orders, (size_t)n_orders * 8, hipMemcpyHostToDevice, st), "upload"))
-                    k_harmonics<<<n_orders, WG, 0, st>>>(dpow, dord, dint, M, dt, omega0);
-                if (hip(hipGetLastError(), "launch") && power)
-                    hip(hipMemcpyAsync(power, dpow, (size_t)M * 8, hipMemcpyDeviceToHost, st), "download");
-                if (rc == GGB_OK && n_orders > 0)
-                    hip(hipMemcpyAsync(intensity, dint, (size_t)n_orders * 8, hipMemcpyDeviceToHost, st),
-                        "download");
-                hip(hipStreamSynchronize(st), "synchronize");
-            }
-        }
+    SpecBufs b = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int rc = GGB_E_HIP;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        g_create_err = "ggb_spectrum: stream creation failed";
+    } else if (dalloc(&b.sig, (size_t)n) != hipSuccess || dalloc(&b.x, (size_t)n) != hipSuccess ||
+               dalloc(&b.pow, (size_t)M) != hipSuccess || dalloc(&b.A, (size_t)L) != hipSuccess ||
+               dalloc(&b.B, (size_t)L) != hipSuccess || dalloc(&b.T, (size_t)L) != hipSuccess ||
+               dalloc(&b.ord, (size_t)(n_orders > 0 ? n_orders : 1)) != hipSuccess ||
+               dalloc(&b.inten, (size_t)(n_orders > 0 ? n_orders : 1)) != hipSuccess) {
+        g_create_err = "ggb_spectrum: device allocation failed";
+        rc = GGB_E_NOMEM;
+    } else {
+        rc = spectrum_on(b, st, signal, n, dt, differentiate, beta, n_pad, omega0, orders, n_orders,
+                         power, intensity, "ggb_spectrum");
     }
-    void *bufs[] = {dsig, dx, dpow, dord, dint, A, B, T};
-    for (void *p : bufs)
-        if (p) (void)hipFree(p);
+    void *bufs[] = {b.sig, b.x, b.pow, b.ord, b.inten, b.A, b.B, b.T};
+    for (void *q : bufs)
+        if (q) (void)hipFree(q);
     if (st) (void)hipStreamDestroy(st);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free spectrum: the buffers of one spectrum live in a file-scope
+// device array of fixed size (each device holds its own copy until the process
+// ends).  Host code never passes the array's name to a kernel: its device
+// address is looked up once per device (hipGetSymbolAddress) and that address
+// is carved into the buffers.  Capacity: FFT length L <= 2^20, so
+// M = (2 n_pad + 1) n <= 2^19 (L >= 2M - 1), and at most 256 harmonic orders.
+// ---------------------------------------------------------------------------
+#define SPEC_NA_L (1L << 20)
+#define SPEC_NA_ORDERS 256
+#define SPEC_NA_DEVICES 64
+static constexpr size_t SPEC_NA_BYTES =
+    3 * (size_t)SPEC_NA_L * sizeof(double2)          // A, B, T
+    + 3 * (size_t)(SPEC_NA_L / 2) * sizeof(double)   // sig, x (n <= M), pow (M <= L / 2)
+    + 2 * (size_t)SPEC_NA_ORDERS * sizeof(double);   // orders, intensities
+static __device__ __align__(256) unsigned char scratch[SPEC_NA_BYTES];
+static void *g_scratch_ptr[SPEC_NA_DEVICES];
+static std::mutex g_scratch_mu;
+
+extern "C" int ggb_spectrum_noalloc(int32_t device, const double *signal, int32_t n, double dt,
+                                    int32_t differentiate, double beta, int32_t n_pad, double omega0,
+                                    const double *orders, int32_t n_orders, double *power,
+                                    double *intensity) {
+    if (!spec_args_ok(signal, n, dt, beta, n_pad, omega0, orders, n_orders, intensity,
+                      "ggb_spectrum_noalloc"))
+        return GGB_E_ARG;
+    const long M = (2L * n_pad + 1) * n;
+    if (spec_fft_length(M) > SPEC_NA_L || n_orders > SPEC_NA_ORDERS) {
+        g_create_err = "ggb_spectrum_noalloc: (2 n_pad + 1) n must be <= 2^19 (FFT length <= 2^20) "
+                       "and n_orders <= 256; use ggb_spectrum for larger spectra";
+        return GGB_E_ARG;
+    }
+    if (device < 0 || device >= SPEC_NA_DEVICES) {
+        g_create_err = "ggb_spectrum_noalloc: device ordinal out of range";
+        return GGB_E_ARG;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        g_create_err = "ggb_spectrum_noalloc: hipSetDevice failed";
+        return GGB_E_HIP;
+    }
+    unsigned char *base;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void *ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr) {
+                g_create_err = std::string("ggb_spectrum_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e);
+                return GGB_E_HIP;
+            }
+            g_scratch_ptr[device] = ptr;
+        }
+        base = static_cast<unsigned char *>(g_scratch_ptr[device]);
+    }
+    SpecBufs b;
+    b.A = reinterpret_cast<double2 *>(base);
+    b.B = b.A + SPEC_NA_L;
+    b.T = b.B + SPEC_NA_L;
+    b.sig = reinterpret_cast<double *>(b.T + SPEC_NA_L);
+    b.x = b.sig + SPEC_NA_L / 2;
+    b.pow = b.x + SPEC_NA_L / 2;
+    b.ord = b.pow + SPEC_NA_L / 2;
+    b.inten = b.ord + SPEC_NA_ORDERS;
+    // every buffer is written before it is read (window, chirps, FFT passes,
+    // power), so nothing needs clearing between calls
+    return spectrum_on(b, nullptr, signal, n, dt, differentiate, beta, n_pad, omega0, orders, n_orders,
+                       power, intensity, "ggb_spectrum_noalloc");
 }
 
 extern "C" int ggb_set_kinetic_variant(ggb_ctx *ctx, int32_t variant) {

@@ -256,6 +256,21 @@ int ggb_spectrum(int32_t device, const double *signal, int32_t n, double dt,
                  int32_t differentiate, double beta, int32_t n_pad, double omega0,
                  const double *orders, int32_t n_orders, double *power, double *intensity);
 
+/* ggb_spectrum without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * results and errors as ggb_spectrum (bit for bit), computed on the legacy
+ * stream in a fixed device scratch of 60 MiB that each device holds from its
+ * first call until the process ends.  Limits: (2 n_pad + 1) n <= 2^19 (FFT
+ * length <= 2^20; e.g. n <= 30840 samples at n_pad = 8) and n_orders <= 256,
+ * else GGB_E_ARG with a message.  All calls on one device share the scratch:
+ * calls that overlap on one device race (serialise them); the scratch keeps
+ * the last call's contents, and every buffer is written before it is read. */
+int ggb_spectrum_noalloc(int32_t device, const double *signal, int32_t n, double dt,
+                         int32_t differentiate, double beta, int32_t n_pad, double omega0,
+                         const double *orders, int32_t n_orders, double *power,
+                         double *intensity);
+
 /* Ground state on the device: the orbitals and v_gs that the propagation
  * starts from.  The reference loads them from workingData/
  * (h_initialiseTDKSOSet.cuh:19-107), written by its diagonalisation SCF
