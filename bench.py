@@ -93,43 +93,112 @@ def _cpu_model():
     return None
 
 
-def batched_run(sysm, pulse, args, fp64_value):
-    """n Ar instances (intensities log-spaced 1e14..3e14) in one context, FP32."""
+FP64_MFMA_TFLOPS = 78.6
+FP32_MFMA_TFLOPS = 157.3
+
+
+def phase_rooflines(sysm, n_inst, precision, phases_us, tiled):
+    """Algorithmic bytes (and matrix-core flops) of each timed phase of one
+    step and the fraction of the MI355X peak they reach in the measured time
+    (phase times from HIP events between the phases: launch gaps included)."""
+    N, Nl, Nth, nh = sysm.N, sysm.n_l, sysm.n_theta, sysm.n_har
+    C = n_inst * sysm.n_orb
+    es = 4 if precision else 8
+    nk = int(sysm.nkeep.sum())
+    h = (Nth + 1) // 2
+    state = Nl * C * N * 16                      # FP64 state, read or written once
+    phase_tab = n_inst * N * Nth * (8 if precision else 16)
+    ang_flops = 2 * 2 * h * Nl * 2 * C * N       # one l<->theta transform pair (parity-split)
+    kin_flops = 2 * nk * 2 * C * N
+    kin_bytes = nk * N * es + state + nk * C * 2 * es
+    work = {
+        "half_potential_laser": (2 * state + phase_tab + (Nl * C * N * 8 if precision and tiled else 0),
+                                 ang_flops),
+        "kinetic_qt": (kin_bytes, kin_flops),
+        "kinetic_q": (kin_bytes, kin_flops),
+        "laser_half_potential_obs": (2 * state + phase_tab + C * N * Nth * es, 1.5 * ang_flops),
+        "density_moments": (C * N * Nth * es + n_inst * N * Nth * 8 + n_inst * nh * N * 8,
+                            2 * nh * Nth * N * n_inst),
+        "hartree": (2 * n_inst * nh * N * 8 + (2 * nh + 1) * N * 8, 0),
+        "dv_phase": (n_inst * nh * N * 8 + 2 * n_inst * N * Nth * 8 + phase_tab,
+                     2 * nh * Nth * N * n_inst),
+    }
+    out = {}
+    for k, (b, f) in work.items():
+        t = phases_us.get(k)
+        if not t:
+            continue
+        peak_tf = FP32_MFMA_TFLOPS if (precision and k in ("kinetic_qt", "kinetic_q",
+                                                             "half_potential_laser",
+                                                             "laser_half_potential_obs")) \
+            else FP64_MFMA_TFLOPS
+        out[k] = {"us": t, "bytes": int(b), "GB/s": b / (t * 1e-6) / 1e9,
+                  "hbm_frac": b / (t * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                  "flops": float(f), "TFLOP/s": f / (t * 1e-6) / 1e12,
+                  "mfma_frac": f / (t * 1e-6) / 1e12 / peak_tf}
+    return out
+
+
+def batched_run(sysm, pulse, args, fp64_value, precision=1):
+    """n Ar instances (intensities log-spaced 1e14..3e14) in one context, FP32
+    mixed form (precision 1) or FP64 (0)."""
     from ggbtddft_amd.engine import Propagator
     from ggbtddft_amd.pulse import Pulse
     n = args.batched
     Is = np.geomspace(1e14, 3e14, n)
-    nst = args.warmup + args.batched_steps
+    nst = args.warmup + args.batched_steps + 100
     E = np.stack([Pulse(I, WAVELENGTH, CYCLES).midpoints(DT)[:nst] for I in Is])
-    prop = Propagator(sysm, n_inst=n, precision=1)
+    prop = Propagator(sysm, n_inst=n, precision=precision)
     prop.set_field(E)
     tuned = prop.autotune(reps=5)
-    log("autotuner (batched fp32): %s %s" % tuned)
+    name = {0: "fp64", 1: "fp32", 2: "fp32 split"}[precision]
+    log("autotuner (batched %s): %s %s" % (name, tuned[0], tuned[1]))
     prop.run(0, args.warmup)
+    prop.set_timing(1)
     t0 = time.perf_counter()
     obs = prop.run(args.warmup, args.batched_steps)
     el = time.perf_counter() - t0
+    tim = prop.timing()
+    prop.set_timing(2)
+    prop.run(args.warmup + args.batched_steps, 50)
+    phases = {k: 1e3 * v[0] / max(v[1], 1) for k, v in prop.timing().items()}
     prop.close()
     assert np.all(np.isfinite(obs["acc"]))
     val = n * sysm.n_orb * args.batched_steps / el
-    log("batched fp32: %d instances, %d steps in %.2f s" % (n, args.batched_steps, el))
+    log("batched %s: %d instances, %d steps in %.2f s" % (name, n, args.batched_steps, el))
+    tiled = tuned[0] in ("tiled16", "tiled32", "split16")
+    kin_us = 1e3 * tim["kinetic_qt"][0] / max(tim["kinetic_qt"][1], 1)
+    C = n * sysm.n_orb
+    # matrix-core flops issued (the split product issues three FP32 products)
+    kin_flops = 2 * int(sysm.nkeep.sum()) * 2 * C * sysm.N * (3 if precision == 2 else 1)
+    peak = FP32_MFMA_TFLOPS if precision else FP64_MFMA_TFLOPS
     return {"value": val, "unit": "orbital-timesteps/s", "instances": n,
             "steps": args.batched_steps, "ms_per_step": 1e3 * el / args.batched_steps,
-            "precision": "fp32 tables/transforms/v_xc, fp64 state and sums",
+            "precision": {0: "fp64 throughout",
+                          1: "fp32 tables/transforms/v_xc, fp64 state and sums",
+                          2: "as fp32, kinetic product compensated (split-FP32: 3 FP32 "
+                             "matrix-core products, FP64 chunk sums)"}[precision],
             "kinetic_variant": tuned[0], "autotuner_ms_per_kinetic_step": tuned[1],
-            "ratio_vs_fp64_plain": val / fp64_value}
+            "ratio_vs_fp64_plain": val / fp64_value,
+            "roofline_kinetic": {"kernel": "kinetic pass 1 (Y = e^{-i lam dt} Q^T psi), %s" % tuned[0],
+                                 "bound": "mfma" if tiled else "hbm",
+                                 "achieved": kin_flops / (kin_us * 1e-6) / 1e12, "peak": peak,
+                                 "unit": "TFLOP/s",
+                                 "frac": kin_flops / (kin_us * 1e-6) / 1e12 / peak,
+                                 "flops_per_launch": kin_flops, "avg_launch_us": kin_us},
+            "phase_us": phases,
+            "phase_rooflines": phase_rooflines(sysm, n, precision, phases, tiled)}
 
 
-def serving_run(sysm_ar, args):
-    """Continuous-batching serving of a Poisson stream of scan requests over the
-    grid range of the serving configuration (He, Ne, two-colour Ar and Kr on
-    N = 256/384/512/768, L_max = 32/48/64/96), FP32 mixed form, arrivals in
-    real time; p50/p99 arrival -> spectrum."""
-    from ggbtddft_amd.serving import Request, Server, poisson_arrivals
-    n = args.serving
-    rng = np.random.default_rng(0)
-    arr = poisson_arrivals(args.serving_rate, n, seed=0)
-    # (atom, N, r_max, L_max, dt, two-colour)
+def serving_requests(n, rate, seed=0):
+    """The serving stream: Poisson arrivals (seed) of mixed requests over the
+    grid range He N=256/L_max=32, Ne 384/48 (dt 0.05), Kr 768/96 (dt 0.02), one
+    colour 800-2000 nm at 5e13-5e14 W/cm^2, and Ar 512/64 (dt 0.02) in situ:
+    800 nm at 1.5e14 W/cm^2 with its second harmonic at 1e-3 relative
+    intensity and a random relative phase; 2-cycle pulses."""
+    from ggbtddft_amd.serving import Request, poisson_arrivals
+    rng = np.random.default_rng(seed)
+    arr = poisson_arrivals(rate, n, seed=seed)
     kinds = [("He", 256, 100.0, 32, 0.05, False), ("Ne", 384, 150.0, 48, 0.05, False),
              (ATOM, N, R_MAX, LMAX, DT, True), ("He", 256, 100.0, 32, 0.05, False),
              ("Ne", 384, 150.0, 48, 0.05, False), ("Kr", 768, 200.0, 96, 0.02, False)]
@@ -140,46 +209,88 @@ def serving_run(sysm_ar, args):
         phase = float(rng.uniform(0.0, 2.0 * np.pi))
         atom, nr, rmax, lmax, dt, two = kinds[k % len(kinds)]
         if two:
-            # in-situ two-colour request: 800 nm at 1.5e14 W/cm^2 with its second
-            # harmonic at 1e-3 relative intensity and a random relative phase
             I, lam, second = 1.5e14, 800.0, (2, 1e-3, phase)
         else:
             second = None
         reqs.append(Request(rid=k, atom=atom, intensity=I, wavelength=lam, n_cycles=2, N=nr,
                             r_max=rmax, lmax=lmax, dt=dt, arrival=float(arr[k]), second=second))
-    srv = Server(slots=args.serving_slots, chunk=args.serving_chunk, ring=1 << 16, precision=1)
-    srv.systems[(ATOM, N, float(R_MAX), LMAX, float(DT))] = sysm_ar
-    t0 = time.perf_counter()
-    for key in {r.key() for r in reqs}:   # ground states and tables before the clock starts
-        srv._system(key)
-    t_sys = time.perf_counter() - t0
+    return reqs
+
+
+def _serve(reqs, slots, chunk, systems=None, warm=True):
+    from ggbtddft_amd.serving import Server
+    srv = Server(slots=slots, chunk=chunk, ring=1 << 16, precision=1)
+    t_sys = 0.0
+    if systems:
+        srv.systems.update(systems)
+    if warm:
+        t0 = time.perf_counter()
+        for key in {r.key() for r in reqs}:   # ground states and tables before the clock starts
+            srv._system(key)
+        t_sys = time.perf_counter() - t0
     results, stats = srv.serve(reqs, realtime=True)
+    systems_out = dict(srv.systems)
     srv.close()
     bad = [r for r in results if r.error is not None or not np.all(np.isfinite(r.spectrum))]
     stats = dict(stats)
-    stats.update({"requests": n, "arrival_rate_per_s": args.serving_rate,
-                  "slots_per_lane": args.serving_slots, "chunk_steps": args.serving_chunk,
-                  "precision": "fp32 mixed",
-                  "workload": "Poisson stream (seed 0): He N=256 Lmax=32, Ne N=384 Lmax=48 "
-                              "(dt=0.05) and Kr N=768 Lmax=96 (dt=0.02), one colour "
-                              "800-2000 nm, 5e13-5e14 W/cm^2; Ar N=512 Lmax=64 (dt=0.02) "
-                              "in situ: 800 nm at 1.5e14 W/cm^2 + 400 nm at 1e-3 relative "
-                              "intensity, random relative phase; 2-cycle pulses",
+    stats.update({"requests": len(reqs), "slots_per_lane": slots, "chunk_steps": chunk,
+                  "precision": "fp32 mixed", "bad_results": len(bad),
                   "system_setup_s_before_clock": t_sys,
-                  "bad_results": len(bad)})
-    log("serving: %d requests, p50 %.2f s, p99 %.2f s" % (n, stats["p50_time_to_spectrum_s"],
-                                                          stats["p99_time_to_spectrum_s"]))
-    return stats
+                  "precompute_inside_clock": not warm})
+    return stats, systems_out
 
 
-def scan_run(n):
+def serving_run(sysm_ar, args):
+    """Continuous-batching serving (FP32 mixed form, arrivals in real time,
+    p50/p99 arrival -> spectrum): the mixed serving stream with the new
+    keys' ground states and tables computed inside the clock (cold) and
+    before it (warm), and a saturated stream of in-situ argon requests on the
+    benchmark grid that keeps every slot of its lane busy."""
+    from ggbtddft_amd.serving import Request, poisson_arrivals
+    out = {}
+    reqs = serving_requests(args.serving, args.serving_rate)
+    desc = ("Poisson stream (seed 0, %.1f/s): He N=256 Lmax=32, Ne N=384 Lmax=48 (dt=0.05) "
+            "and Kr N=768 Lmax=96 (dt=0.02), one colour 800-2000 nm, 5e13-5e14 W/cm^2; "
+            "Ar N=512 Lmax=64 (dt=0.02) in situ: 800 nm at 1.5e14 W/cm^2 + 400 nm at 1e-3 "
+            "relative intensity, random relative phase; 2-cycle pulses" % args.serving_rate)
+    cold, systems = _serve(reqs, args.serving_slots, args.serving_chunk,
+                           systems={(ATOM, N, float(R_MAX), LMAX, float(DT)): sysm_ar}, warm=False)
+    cold["workload"] = desc + " (ground states and tables of He, Ne, Kr computed inside the clock)"
+    log("serving cold: p50 %.2f s, p99 %.2f s" % (cold["p50_time_to_spectrum_s"],
+                                                  cold["p99_time_to_spectrum_s"]))
+    warm, _ = _serve(reqs, args.serving_slots, args.serving_chunk, systems=systems, warm=True)
+    warm["workload"] = desc + " (ground states and tables before the clock)"
+    log("serving warm: p50 %.2f s, p99 %.2f s" % (warm["p50_time_to_spectrum_s"],
+                                                  warm["p99_time_to_spectrum_s"]))
+    out["mixed_stream_warm"] = warm
+    out["mixed_stream_cold"] = cold
+    if args.serving_saturated > 0:
+        n = args.serving_saturated
+        arr = poisson_arrivals(100.0, n, seed=1)
+        rng = np.random.default_rng(1)
+        sat = [Request(rid=k, atom=ATOM, intensity=1.5e14, wavelength=800.0, n_cycles=2, N=N,
+                       r_max=R_MAX, lmax=LMAX, dt=DT, arrival=float(arr[k]),
+                       second=(2, 1e-3, float(rng.uniform(0.0, 2.0 * np.pi)))) for k in range(n)]
+        st, _ = _serve(sat, n, args.serving_chunk, systems={(ATOM, N, float(R_MAX), LMAX, float(DT)): sysm_ar})
+        st["workload"] = ("%d in-situ Ar requests (N=512 Lmax=64 dt=0.02, 800 nm 2 cycles at "
+                          "1.5e14 W/cm^2 + 400 nm at 1e-3, random phase), Poisson arrivals at "
+                          "100/s, one lane of %d slots" % (n, n))
+        log("serving saturated: %.0f orbital-timesteps/s, p50 %.2f s, p99 %.2f s"
+            % (st["orbital_timesteps_per_s"], st["p50_time_to_spectrum_s"],
+               st["p99_time_to_spectrum_s"]))
+        out["saturated_argon"] = st
+    return out
+
+
+def scan_run(n, cycle_scale=1.0):
     """The batched scan of n instances served as one batch (all requests at
     t = 0, one lane per grid key, each lane with a slot per request of its
     key, idle slots skipped once their request is done), FP64 then FP32 mixed;
-    orbital-timesteps/s over the whole scan and the FP32 error per instance."""
+    orbital-timesteps/s over the whole scan and the FP32 error per instance.
+    cycle_scale < 1 shortens every pulse (same grids, uneven step counts)."""
     from ggbtddft_amd.scan import scan_requests
     from ggbtddft_amd.serving import Server
-    reqs = scan_requests(n)
+    reqs = scan_requests(n, cycle_scale=cycle_scale)
     keys = {}
     for r in reqs:
         keys[r.key()] = keys.get(r.key(), 0) + 1
@@ -194,8 +305,9 @@ def scan_run(n):
            "steps_min": int(min(n_steps)), "steps_max": int(max(n_steps)),
            "system_setup_s_before_clock": t_sys,
            "workload": "batched scan (seed 0): He/Ne, 5e13-5e14 W/cm^2 log-uniform, "
-                       "600-2000 nm, 4-16 cycles, grid by quiver radius "
-                       "(N 256/384/512, L_max 32/48/64), dt 0.05, all requests at t = 0"}
+                       "600-2000 nm, 4-16 cycles%s, grid by quiver radius "
+                       "(N 256/384/512, L_max 32/48/64), dt 0.05, all requests at t = 0"
+                       % ("" if cycle_scale == 1.0 else " scaled by %g" % cycle_scale)}
     res = {}
     for prec, name in ((0, "fp64"), (1, "fp32")):
         log("scan %s: serving %d requests on %d lanes" % (name, n, len(keys)))
@@ -282,10 +394,11 @@ def measured_traffic():
 
 def precision_report(sysm, pulse, n_batch=16):
     """Full-length runs: FP64 one instance, FP32 mixed one instance (narrow
-    kinetic kernels) and FP32 mixed in a batch of n_batch instances (tiled
+    kinetic kernels), FP32 mixed in a batch of n_batch instances (tiled
     kinetic kernels, the configuration of the batched leg; instance 0 carries
-    the benchmark pulse, the others 1e14-3e14 W/cm^2); errors of the FP32
-    spectra against FP64."""
+    the benchmark pulse, the others 1e14-3e14 W/cm^2) and the same batch with
+    the compensated split-FP32 kinetic product; errors of the FP32 spectra
+    against FP64."""
     from ggbtddft_amd import spectrum as sp
     from ggbtddft_amd.engine import Propagator
     from ggbtddft_amd.pulse import Pulse
@@ -296,7 +409,8 @@ def precision_report(sysm, pulse, n_batch=16):
     fields = [E] + [Pulse(float(I), WAVELENGTH, CYCLES).midpoints(DT)[:n]
                     for I in rng.uniform(1e14, 3e14, n_batch - 1)]
     runs = {}
-    for name, prec, ninst in (("fp64", 0, 1), ("fp32", 1, 1), ("fp32_batched", 1, n_batch)):
+    for name, prec, ninst in (("fp64", 0, 1), ("fp32", 1, 1), ("fp32_batched", 1, n_batch),
+                              ("fp32_split_batched", 2, n_batch)):
         t0 = time.perf_counter()
         with Propagator(sysm, n_inst=ninst, precision=prec) as prop:
             prop.set_field(np.stack(fields[:ninst]) if ninst > 1 else E)
@@ -318,7 +432,7 @@ def precision_report(sysm, pulse, n_batch=16):
     rep = {"steps": n, "cutoff_order": cut, "orders": orders.tolist(),
            "one_percent_in_log10": float(np.log10(1.01)), "wall_s_fp64": runs["fp64"][2],
            "batch_instances": n_batch}
-    for name in ("fp32", "fp32_batched"):
+    for name in ("fp32", "fp32_batched", "fp32_split_batched"):
         a32, d32, wall = runs[name]
         _, S32 = sp.acceleration_spectrum(a32, DT)
         _, D32 = sp.dipole_spectrum(d32, DT)
@@ -341,16 +455,28 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=100)
-    ap.add_argument("--scan", type=int, default=0, metavar="N",
-                    help="also serve the batched He/Ne scan of N instances, FP64 then FP32 "
-                         "(64 is the scan configuration; minutes of GPU time)")
+    ap.add_argument("--scan", type=int, default=64, metavar="N",
+                    help="serve the batched He/Ne scan of N instances, FP64 then FP32 "
+                         "(64 is the scan configuration; 0 = skip)")
+    ap.add_argument("--scan-cycle-scale", type=float, default=1.0 / 16,
+                    help="shorten every scan pulse by this factor (1 = the full scan, "
+                         "minutes of GPU time)")
     ap.add_argument("--batched", type=int, default=64,
                     help="Ar instances for the batched FP32 leg (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--no-batched-fp64", action="store_true",
+                    help="skip the FP64 leg at the batched leg's instance count")
+    ap.add_argument("--no-batched-split", action="store_true",
+                    help="skip the compensated split-FP32 kinetic leg at that instance count")
+    ap.add_argument("--fp64-window", type=int, default=1000,
+                    help="steps of the FP64 window the batched ratios divide by (at least "
+                         "this many, whatever --steps is)")
     ap.add_argument("--serving", type=int, default=24,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-slots", type=int, default=8)
+    ap.add_argument("--serving-saturated", type=int, default=64,
+                    help="in-situ argon requests of the saturated serving run (0 = skip)")
     ap.add_argument("--serving-chunk", type=int, default=256,
                     help="steps per lane between seatings (a multiple of 64 keeps graph replay whole)")
     ap.add_argument("--no-precision-report", action="store_true",
@@ -385,7 +511,7 @@ def main():
         % (time.perf_counter() - t0, int(sysm.nkeep.sum())))
     pulse = Pulse(INTENSITY, WAVELENGTH, CYCLES)
     nfull = pulse.n_steps(DT)
-    need = args.warmup + args.steps
+    need = args.warmup + args.steps + args.fp64_window + 200
     if need > nfull:
         raise SystemExit("warmup + steps (%d) exceeds the pulse (%d steps)" % (need, nfull))
     E = pulse.midpoints(DT)
@@ -403,10 +529,21 @@ def main():
     log("timed %d steps in %.2f s" % (args.steps, el))
     tim = prop.timing()
     psi = prop.download_orbitals()
+    s_next = args.warmup + args.steps
+    # the denominator of the batched ratios: a window of at least fp64_window steps
+    if args.steps >= args.fp64_window:
+        ref_steps, ref_el = args.steps, el
+    else:
+        prop.set_timing(0)
+        t0 = time.perf_counter()
+        prop.run(s_next, args.fp64_window)
+        ref_steps, ref_el = args.fp64_window, time.perf_counter() - t0
+        s_next += args.fp64_window
+    fp64_ref = sysm.n_orb * ref_steps / ref_el
     # per-phase breakdown in a separate pass (events between every phase)
     prop.set_timing(2)
-    nb = min(200, nfull - need)
-    prop.run(need, nb)
+    nb = 200
+    prop.run(s_next, nb)
     phases = {k: v[0] / max(v[1], 1) for k, v in prop.timing().items()}
     prop.close()
     assert np.all(np.isfinite(obs["acc"])) and np.all(np.isfinite(psi))
@@ -444,6 +581,9 @@ def main():
                    "pulse": "800 nm sin^2 8 cycles 2e14 W/cm^2", "full_run_steps": nfull,
                    "precision": "fp64", "parallelism": "single GPU"},
         "phase_ms_per_step": phases,
+        "phase_rooflines": phase_rooflines(sysm, 1, 0, {k: 1e3 * v for k, v in phases.items()}, False),
+        "fp64_ratio_denominator": {"value": fp64_ref, "steps": ref_steps,
+                                   "ms_per_step": 1e3 * ref_el / ref_steps},
         "kinetic_variant": tuned[0], "autotuner_ms_per_kinetic_step": tuned[1],
         "roofline": {"kernel": "k_kin_fwd (kinetic propagator, Q^T pass)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -454,11 +594,20 @@ def main():
                      "avg_launch_ms_q_pass": avg_q},
     }
     if args.batched > 0:
-        line["batched_fp32"] = batched_run(sysm, pulse, args, value)
+        line["batched_fp32"] = batched_run(sysm, pulse, args, fp64_ref, precision=1)
+        if not args.no_batched_fp64:
+            b64 = batched_run(sysm, pulse, args, fp64_ref, precision=0)
+            line["batched_fp64"] = b64
+            line["batched_fp32"]["ratio_vs_fp64_batched"] = line["batched_fp32"]["value"] / b64["value"]
+        if not args.no_batched_split:
+            line["batched_fp32_split"] = batched_run(sysm, pulse, args, fp64_ref, precision=2)
     if args.serving > 0:
         line["serving"] = serving_run(sysm, args)
+        sat = line["serving"].get("saturated_argon")
+        if sat:
+            sat["ratio_vs_fp64_plain"] = sat["orbital_timesteps_per_s"] / fp64_ref
     if args.scan > 0:
-        line["scan"] = scan_run(args.scan)
+        line["scan"] = scan_run(args.scan, args.scan_cycle_scale)
     line["spectrum"] = spectrum_timing(nfull)
     line["ground_state"] = gs_stats
     if not args.no_precision_report:

@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libggbtd.so")
 
-GGB_ABI_VERSION = 3
+GGB_ABI_VERSION = 4
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "call out of order",
           -4: "device allocation failed"}
 
@@ -23,10 +23,10 @@ EXPORTS = ("ggb_abi_version", "ggb_create", "ggb_upload_static", "ggb_upload_orb
            "ggb_spectrum", "ggb_set_kinetic_variant", "ggb_autotune", "ggb_ground_state",
            "ggb_set_channels", "ggb_get_channels", "ggb_spectrum_noalloc", "ggb_last_error",
            "ggb_destroy")
-KIN_VARIANTS = ("default", "narrow", "tiled16", "tiled32")
-NPHASE = 5
+KIN_VARIANTS = ("default", "narrow", "tiled16", "tiled32", "split16")
+NPHASE = 7
 PHASES = ("half_potential_laser", "kinetic_qt", "kinetic_q", "laser_half_potential_obs",
-          "potential_rebuild")
+          "density_moments", "hartree", "dv_phase")
 
 
 class Problem(ctypes.Structure):

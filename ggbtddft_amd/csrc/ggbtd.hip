@@ -13,16 +13,17 @@
 // dipoleAcceleration.cuh) as fused reductions.
 //
 // Kernels, launch order per step:
-//   k_angular<mode 0>   Vh, Le, Lo                 grid (ceil(N/8), C)
-//   k_kin_fwd           Y = e^{-i lam dt} Q^T psi   grid (nk/64, 2C/16, n_l)
-//   k_kin_bwd           psi = Q Y                   grid (N/64,  2C/16, n_l)
+//   k_angular<mode 0>   Vh, Le, Lo                       grid (N / 8 NT, C)
+//   k_kin_fwd(_tiled)   Y = e^{-i lam dt} Q^T psi        grid (nk/16, 2C/16, n_l) or 64 x 128 tiles
+//   k_kin_bwd(_tiled)   psi = Q Y                        grid (N/16,  2C/16, n_l) or 64 x 128 tiles
 //   k_angular<mode 1>   Lo, Le, Vh, |B psi|^2, observable partials
-//   k_obs_reduce        per-column observable sums
-//   k_density           rho_u, v_xc, multipole moments q_L   grid (ceil(N/8), n_inst)
-//   k_hartree           V_L by prefix/suffix affine scans   grid (n_har, n_inst)
-//   k_dv                dV and the phase factors of the next step
+//   k_rho               rho_u = sum occ |Psi|^2, v_xc; observable sums
+//   k_tile_gemm<Qmom>   multipole moments q_L = P_L^T rho_u
+//   k_hartree           V_L by prefix/suffix affine scans  grid (n_har, n_inst)
+//   k_tile_gemm<Dv>     dV = sum_L V_L P_L, v_xc - v_gs, phase factors of the next step
 //
-// All matrix products run on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+// The matrix products run on the matrix cores: v_mfma_f64_16x16x4f64, and
+// v_mfma_f32_16x16x4f32 for the FP32 products of the mixed form.
 // Lane layouts (checked on the device by ggb_selftest_mfma):
 //   A (16x4):  lane L holds A[L % 16][L / 16]
 //   B (4x16):  lane L holds B[L / 16][L % 16]
@@ -603,6 +604,280 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
                     double v = (double)hi[x][y][r];
                     if (rrow) v += rc * rrow[i];
                     Xl[((size_t)c * N + i) * 2 + p] = v;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compensated split-FP32 kinetic passes (precision 2).  Every FP64 operand is
+// split into FP32 parts, v = v_hi + v_lo with v_hi = fl32(v), v_lo =
+// fl32(v - v_hi), and a product a b is formed as a_hi b_hi + a_hi b_lo +
+// a_lo b_hi on the FP32 matrix cores (the dropped a_lo b_lo is 2^-48 of
+// a b); the main and correction products go to separate FP32 accumulators,
+// whose chunk sums (16 values of K) are added in FP64.  Tiles, staging and
+// XCD order as the tiled passes above (64 x 128 work-group tiles, K chunks of
+// 16 double buffered in LDS).  Pass 1 splits the FP64 state minus its
+// ground-state component itself; Y leaves pass 1 as (Y_hi, Y_lo).
+// ---------------------------------------------------------------------------
+#define SPK 16
+static __device__ __forceinline__ void split32(double v, float &hi, float &lo) {
+    hi = (float)v;
+    lo = (float)(v - (double)hi);
+}
+
+// 3 x 8 products of one K step of 4 for the wave's 2 x 4 tiles
+static __device__ __forceinline__ void kin_split_step(const float (*Ah)[TMP], const float (*Al)[TMP],
+                                                      const float (*Bh)[TNP], const float (*Bl)[TNP],
+                                                      int kk, int wm, int wn, int lc, f4_t (&acc)[2][4],
+                                                      f4_t (&cor)[2][4]) {
+    const float ah0 = Ah[kk][wm + lc], ah1 = Ah[kk][wm + 16 + lc];
+    const float al0 = Al[kk][wm + lc], al1 = Al[kk][wm + 16 + lc];
+    float bh[4], bl[4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) bh[y] = Bh[kk][wn + 16 * y + lc], bl[y] = Bl[kk][wn + 16 * y + lc];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        acc[0][y] = mfma_t<float>::op(ah0, bh[y], acc[0][y]);
+        acc[1][y] = mfma_t<float>::op(ah1, bh[y], acc[1][y]);
+        cor[0][y] = mfma_t<float>::op(ah0, bl[y], cor[0][y]);
+        cor[1][y] = mfma_t<float>::op(ah1, bl[y], cor[1][y]);
+        cor[0][y] = mfma_t<float>::op(al0, bh[y], cor[0][y]);
+        cor[1][y] = mfma_t<float>::op(al1, bh[y], cor[1][y]);
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_kin_fwd_split(const float *__restrict__ Qh,
+                                                      const float *__restrict__ Ql,
+                                                      const double2 *__restrict__ kph,
+                                                      const int32_t *__restrict__ nkeep,
+                                                      const double *__restrict__ psi, KinRef ref,
+                                                      float *__restrict__ Yh, float *__restrict__ Yl,
+                                                      int N, int nk_max, int C, int n_orb) {
+    constexpr int AV = SPK * TM / WG;  // A values per thread (consecutive a)
+    constexpr int BV = SPK * 64 / WG;  // complex B values per thread (consecutive i)
+    __shared__ float Ash[2][SPK][TMP], Asl[2][SPK][TMP];  // [k = i][m = a]
+    __shared__ float Bsh[2][SPK][TNP], Bsl[2][SPK][TNP];  // [k = i][n = real column]
+    const tile3 tb = xcd_tile();
+    const int l = tb.z;
+    const int nk = nkeep[l];
+    const int a0 = tb.x * TM;
+    if (a0 >= nk) return;  // block-uniform
+    const int n0 = tb.y * TN;
+    if (!kin_cols_active(ref, n0, TN, C, n_orb)) return;  // idle instances only
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    const size_t qoff = (size_t)l * N * nk_max;
+    const double2 *Xl = (const double2 *)psi + (size_t)l * C * N;
+    const int ar = t / (TM / AV), am = (t % (TM / AV)) * AV;
+    const int bc = t / (SPK / BV), bk = (t % (SPK / BV)) * BV;
+    const int cg = (n0 >> 1) + bc;
+    const bool cok = cg < C;
+    const bool aok = a0 + am < nk_max;
+    // ground-state component of this thread's column in block l (both parts)
+    double cre = 0.0, cim = 0.0;
+    const double *rre = cok ? kin_ref_row(ref, l, cg, 0, N, n_orb, false, &cre) : nullptr;
+    if (rre) (void)kin_ref_row(ref, l, cg, 1, N, n_orb, false, &cim);
+    float rah[AV], ral[AV], rbh[2 * BV], rbl[2 * BV];
+    auto load = [&](int k0) {
+        const int i = k0 + ar;
+        const bool ok = aok && i < N;
+#pragma unroll
+        for (int u = 0; u < AV; ++u) {
+            const size_t o = qoff + (size_t)i * nk_max + a0 + am + u;
+            rah[u] = ok ? Qh[o] : 0.f;
+            ral[u] = ok ? Ql[o] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < BV; ++u) {
+            const int ii = k0 + bk + u;
+            double2 v = (cok && ii < N) ? Xl[(size_t)cg * N + ii] : make_double2(0.0, 0.0);
+            if (rre && ii < N) {
+                v.x -= cre * rre[ii];
+                v.y -= cim * rre[ii];
+            }
+            split32(v.x, rbh[2 * u], rbl[2 * u]);
+            split32(v.y, rbh[2 * u + 1], rbl[2 * u + 1]);
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < AV; ++u) Ash[buf][ar][am + u] = rah[u], Asl[buf][ar][am + u] = ral[u];
+#pragma unroll
+        for (int u = 0; u < BV; ++u) {
+            Bsh[buf][bk + u][2 * bc] = rbh[2 * u];
+            Bsh[buf][bk + u][2 * bc + 1] = rbh[2 * u + 1];
+            Bsl[buf][bk + u][2 * bc] = rbl[2 * u];
+            Bsl[buf][bk + u][2 * bc + 1] = rbl[2 * u + 1];
+        }
+    };
+    f4_t acc[2][4], cor[2][4];
+    double hi[2][4][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            acc[x][y] = (f4_t){0, 0, 0, 0};
+            cor[x][y] = (f4_t){0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hi[x][y][r] = 0.0;
+        }
+    load(0);
+    stash(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < N; k0 += SPK) {
+        const bool more = k0 + SPK < N;
+        if (more) load(k0 + SPK);
+#pragma unroll
+        for (int ks = 0; ks < SPK / 4; ++ks)
+            kin_split_step(Ash[buf], Asl[buf], Bsh[buf], Bsl[buf], ks * 4 + kr, wm, wn, lc, acc, cor);
+        if (more) stash(buf ^ 1);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hi[x][y][r] += (double)acc[x][y][r] + (double)cor[x][y][r];
+                acc[x][y] = (f4_t){0, 0, 0, 0};
+                cor[x][y] = (f4_t){0, 0, 0, 0};
+            }
+        __syncthreads();
+        buf ^= 1;
+    }
+    const double2 *kphl = kph + (size_t)l * nk_max;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int col = n0 + wn + 16 * y + lc;
+            const int c = col >> 1, p = col & 1;
+            const int ab = a0 + wm + 16 * x;
+            double o[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = ab + mfma_t<float>::row(kr, r);
+                const double v = hi[x][y][r];
+                const double vo = __shfl_xor(v, 1);
+                o[r] = kin_factor<double>(v, vo, p, kphl[a < nk ? a : 0]);
+            }
+            if (c < C) {
+                const size_t yo = (((size_t)l * C + c) * 2 + p) * nk_max;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int a = ab + mfma_t<float>::row(kr, r);
+                    if (a < nk) split32(o[r], Yh[yo + a], Yl[yo + a]);
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_kin_bwd_split(const float *__restrict__ Qh,
+                                                      const float *__restrict__ Ql,
+                                                      const int32_t *__restrict__ nkeep,
+                                                      const float *__restrict__ Yh,
+                                                      const float *__restrict__ Yl, KinRef ref,
+                                                      double *__restrict__ psi, int N, int nk_max,
+                                                      int C, int n_orb) {
+    constexpr int AV = SPK * TM / WG;  // A values per thread (consecutive a of one row i)
+    constexpr int BV = SPK * TN / WG;  // B values per thread (consecutive a of one column)
+    __shared__ float Ash[2][SPK][TMP], Asl[2][SPK][TMP];  // [k = a][m = i]
+    __shared__ float Bsh[2][SPK][TNP], Bsl[2][SPK][TNP];  // [k = a][n = real column]
+    const tile3 tb = xcd_tile();
+    const int l = tb.z;
+    const int nk = nkeep[l];
+    const int i0 = tb.x * TM;
+    const int n0 = tb.y * TN;
+    if (!kin_cols_active(ref, n0, TN, C, n_orb)) return;  // idle instances only
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    const size_t qoff = (size_t)l * N * nk_max;
+    const size_t yoff = (size_t)l * C * 2 * nk_max;
+    const int am = t / (SPK / AV), ak = (t % (SPK / AV)) * AV;
+    const int bn = t / (SPK / BV), bk = (t % (SPK / BV)) * BV;
+    const int ia = i0 + am;
+    const int colb = n0 + bn;
+    const bool bok = (colb >> 1) < C;
+    // the tables and Y are zero from nkeep up to nk_max: runs of AV, BV (<= 16,
+    // dividing 16) are wholly inside or outside [0, nk_max)
+    float rah[AV], ral[AV], rbh[BV], rbl[BV];
+    auto load = [&](int k0) {
+        const bool oka = ia < N && k0 + ak < nk_max;
+#pragma unroll
+        for (int u = 0; u < AV; ++u) {
+            const size_t o = qoff + (size_t)ia * nk_max + k0 + ak + u;
+            rah[u] = oka ? Qh[o] : 0.f;
+            ral[u] = oka ? Ql[o] : 0.f;
+        }
+        const bool okb = bok && k0 + bk < nk_max;
+#pragma unroll
+        for (int u = 0; u < BV; ++u) {
+            const size_t o = yoff + (size_t)colb * nk_max + k0 + bk + u;
+            rbh[u] = okb ? Yh[o] : 0.f;
+            rbl[u] = okb ? Yl[o] : 0.f;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < AV; ++u) Ash[buf][ak + u][am] = rah[u], Asl[buf][ak + u][am] = ral[u];
+#pragma unroll
+        for (int u = 0; u < BV; ++u) Bsh[buf][bk + u][bn] = rbh[u], Bsl[buf][bk + u][bn] = rbl[u];
+    };
+    f4_t acc[2][4], cor[2][4];
+    double hi[2][4][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            acc[x][y] = (f4_t){0, 0, 0, 0};
+            cor[x][y] = (f4_t){0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hi[x][y][r] = 0.0;
+        }
+    load(0);
+    stash(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < nk; k0 += SPK) {
+        const bool more = k0 + SPK < nk;
+        if (more) load(k0 + SPK);
+#pragma unroll
+        for (int ks = 0; ks < SPK / 4; ++ks)
+            kin_split_step(Ash[buf], Asl[buf], Bsh[buf], Bsl[buf], ks * 4 + kr, wm, wn, lc, acc, cor);
+        if (more) stash(buf ^ 1);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hi[x][y][r] += (double)acc[x][y][r] + (double)cor[x][y][r];
+                acc[x][y] = (f4_t){0, 0, 0, 0};
+                cor[x][y] = (f4_t){0, 0, 0, 0};
+            }
+        __syncthreads();
+        buf ^= 1;
+    }
+    double *Xo = psi + (size_t)l * C * N * 2;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int col = n0 + wn + 16 * y + lc;
+            const int c = col >> 1, p = col & 1;
+            if (c >= C) continue;
+            double rc = 0.0;
+            const double *rrow = kin_ref_row(ref, l, c, p, N, n_orb, true, &rc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wm + 16 * x + mfma_t<float>::row(kr, r);
+                if (i < N) {
+                    double v = hi[x][y][r];
+                    if (rrow) v += rc * rrow[i];
+                    Xo[((size_t)c * N + i) * 2 + p] = v;
                 }
             }
         }
@@ -1644,6 +1919,7 @@ struct ggb_ctx {
     int kin_variant;  // GGB_KIN_*
     bool have_ref;
     float2 *psi32;  // FP32 copy of the state for the tiled pass 1 (mixed form, C > 8)
+    float *Qlo, *Ylo;  // precision 2: low FP32 parts of Q and of Y
     double *phi_ref, *eps;
     int32_t *l0;
     double4 *ab;
@@ -1710,7 +1986,7 @@ extern "C" void ggb_destroy(ggb_ctx *ctx) {
                     ctx->r, ctx->hpow, ctx->wr, ctx->v_gs, ctx->mask, ctx->occ, ctx->rho_frozen,
                     ctx->nkeep, ctx->mcol, ctx->dens, ctx->vxc, ctx->rho_u, ctx->q, ctx->V, ctx->dv,
                     ctx->obs_part, ctx->obs, ctx->E, ctx->phase, ctx->phi_ref, ctx->eps,
-                    ctx->l0, ctx->ab, ctx->psi32, ctx->active, ctx->chan};
+                    ctx->l0, ctx->ab, ctx->psi32, ctx->active, ctx->chan, ctx->Qlo, ctx->Ylo};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -1727,7 +2003,9 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     d += Nl * C * N * 2;                 // psi
     const size_t es = p->precision ? 4 : 8;
     const size_t nblk = ang_geometry(p).nblk;
-    size_t tbytes = (Nl * C * nk * 2 + 2 * Nl * N * nk + (size_t)p->n_m * nblk) * es;  // Y, Q, QT, B
+    // Y, Q, QT (not in precision 2), B; precision 2 adds the low parts of Q and Y
+    size_t tbytes = (Nl * C * nk * 2 + (p->precision == 2 ? 1 : 2) * Nl * N * nk + (size_t)p->n_m * nblk) * es;
+    if (p->precision == 2) tbytes += (Nl * N * nk + Nl * C * nk * 2) * sizeof(float);
     d += 2 * Nl * nk;                    // e^{-i lam dt}
     d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
     d += 5 * N + p->n_orb;               // r, wr, v_gs, mask, rho_frozen, occ
@@ -1739,7 +2017,7 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     d += C * (size_t)ntiles * 3;         // obs_part
     if (p->precision) {
         d += (size_t)p->n_orb * (N + 1) + 4 * C;                 // phi_ref, eps, ab
-        tbytes += Nl * C * N * 2 * 4;                          // psi32
+        if (p->precision == 1) tbytes += Nl * C * N * 2 * 4;   // psi32
     }
     return d * sizeof(double) + tbytes + (Nl + 2 * p->n_orb) * sizeof(int32_t);
 }
@@ -1765,7 +2043,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
         p->n_theta > 256 || p->n_har < 1 || p->n_har > 256 || p->n_orb < 1 || p->n_inst < 1 ||
         p->n_m < 1 || p->n_m > p->n_l || p->nk_max < 16 || p->nk_max % 16 != 0 ||
         p->nk_max > ((p->n_r + 15) / 16) * 16 || !(p->dt > 0.0) ||
-        (p->precision != 0 && p->precision != 1) ||
+        (p->precision < 0 || p->precision > 2) ||
         (long)p->n_orb * p->n_inst > 65535) {
         g_create_err = "ggb_create: problem sizes out of range";
         return GGB_E_ARG;
@@ -1793,6 +2071,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->kin_variant = GGB_KIN_DEFAULT;
     ctx->have_ref = false;
     ctx->psi32 = nullptr;
+    ctx->Qlo = ctx->Ylo = nullptr;
     ctx->phi_ref = ctx->eps = nullptr;
     ctx->l0 = nullptr;
     ctx->ab = nullptr;
@@ -1823,7 +2102,8 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     const size_t nB = (size_t)p->n_m * ctx->ang.nblk;
     ok &= dalloc(&ctx->Y, (nY * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->Q, (nQ * es + 7) / 8) == hipSuccess;
-    ok &= dalloc(&ctx->QT, (nQ * es + 7) / 8) == hipSuccess;  // [l][a][i] for pass 2
+    if (p->precision != 2)  // [l][a][i] for the narrow pass 2 (precision 2 runs the split passes only)
+        ok &= dalloc(&ctx->QT, (nQ * es + 7) / 8) == hipSuccess;
     ok &= dalloc(&ctx->kph, Nl * (size_t)p->nk_max) == hipSuccess;
     ok &= dalloc(&ctx->nkeep, Nl) == hipSuccess;
     ok &= dalloc(&ctx->B, (nB * es + 7) / 8) == hipSuccess;
@@ -1852,12 +2132,18 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
         ok &= dalloc(&ctx->eps, (size_t)p->n_orb) == hipSuccess;
         ok &= dalloc(&ctx->l0, (size_t)p->n_orb) == hipSuccess;
         ok &= dalloc(&ctx->ab, C) == hipSuccess;
-        ok &= dalloc(&ctx->psi32, Nl * C * N) == hipSuccess;  // tiled pass 1 (any C, autotuner)
+        if (p->precision == 1) ok &= dalloc(&ctx->psi32, Nl * C * N) == hipSuccess;  // tiled pass 1
+        if (p->precision == 2) {
+            ok &= dalloc(&ctx->Qlo, nQ) == hipSuccess;
+            ok &= dalloc(&ctx->Ylo, nY) == hipSuccess;
+        }
     }
     if (!ok) return bail("ggb_create: device allocation failed", GGB_E_NOMEM);
     // zero everything that kernels may read before it is written
     if (hipMemset(ctx->B, 0, nB * es) != hipSuccess || hipMemset(ctx->Q, 0, nQ * es) != hipSuccess ||
-        hipMemset(ctx->QT, 0, nQ * es) != hipSuccess ||
+        (ctx->QT && hipMemset(ctx->QT, 0, nQ * es) != hipSuccess) ||
+        (ctx->Qlo && hipMemset(ctx->Qlo, 0, nQ * sizeof(float)) != hipSuccess) ||
+        (ctx->Ylo && hipMemset(ctx->Ylo, 0, nY * sizeof(float)) != hipSuccess) ||
         hipMemset(ctx->kph, 0, Nl * (size_t)p->nk_max * sizeof(double2)) != hipSuccess ||
         hipMemset(ctx->Y, 0, nY * es) != hipSuccess ||
         hipMemset(ctx->psi, 0, Nl * C * N * 2 * sizeof(double)) != hipSuccess ||
@@ -1920,13 +2206,21 @@ extern "C" int ggb_upload_static(ggb_ctx *ctx, const ggb_static *s) {
                     for (size_t a = 0; a < nk; ++a) qt[(l * nk + a) * N + i] = s->Q[(l * N + i) * nk + a];
             HIPCHK(ctx, hipMemcpy(ctx->QT, qt.data(), nQ * 8, hipMemcpyHostToDevice));
         } else {
-            std::vector<float> q32(nQ), qt(nQ);
+            std::vector<float> q32(nQ);
             for (size_t k = 0; k < nQ; ++k) q32[k] = (float)s->Q[k];
-            for (size_t l = 0; l < Nl; ++l)
-                for (size_t i = 0; i < N; ++i)
-                    for (size_t a = 0; a < nk; ++a) qt[(l * nk + a) * N + i] = q32[(l * N + i) * nk + a];
             HIPCHK(ctx, hipMemcpy(ctx->Q, q32.data(), nQ * 4, hipMemcpyHostToDevice));
-            HIPCHK(ctx, hipMemcpy(ctx->QT, qt.data(), nQ * 4, hipMemcpyHostToDevice));
+            if (ctx->QT) {
+                std::vector<float> qt(nQ);
+                for (size_t l = 0; l < Nl; ++l)
+                    for (size_t i = 0; i < N; ++i)
+                        for (size_t a = 0; a < nk; ++a) qt[(l * nk + a) * N + i] = q32[(l * N + i) * nk + a];
+                HIPCHK(ctx, hipMemcpy(ctx->QT, qt.data(), nQ * 4, hipMemcpyHostToDevice));
+            }
+            if (ctx->Qlo) {  // low parts: Q - fl32(Q), rounded to FP32
+                std::vector<float> lo(nQ);
+                for (size_t k = 0; k < nQ; ++k) lo[k] = (float)(s->Q[k] - (double)q32[k]);
+                HIPCHK(ctx, hipMemcpy(ctx->Qlo, lo.data(), nQ * 4, hipMemcpyHostToDevice));
+            }
         }
     }
     {
@@ -2086,7 +2380,8 @@ template <int MODE>
 static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
     AngArgs g = ang_args(ctx, step);
     g.cur = cur;
-    if (MODE == 0 && ctx->p.precision && kin_variant(ctx) >= GGB_KIN_TILED16) {
+    if (MODE == 0 && ctx->psi32 &&
+        (kin_variant(ctx) == GGB_KIN_TILED16 || kin_variant(ctx) == GGB_KIN_TILED32)) {
         g.psi32 = ctx->psi32;  // the kinetic pass 1 that follows reads it
         g.l0 = ctx->have_ref ? ctx->l0 : nullptr;
     }
@@ -2099,7 +2394,11 @@ static void launch_angular(ggb_ctx *ctx, int step, const int *cur = nullptr) {
 // potential rebuild from the |Psi|^2 tiles already in ctx->dens
 // ob: the observable reduction of the step, folded into k_rho (interacting) or
 // launched on its own
-static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
+static hipError_t no_mark_k(int) { return hipSuccess; }
+
+// mk(0) after the density kernel, mk(1) after the Hartree scans (timing events)
+template <typename MarkK = hipError_t (*)(int)>
+static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr, MarkK mk = no_mark_k) {
     const ggb_problem &p = ctx->p;
     hipStream_t st = ctx->stream;
     const int N = p.n_r, Nth = p.n_theta, nh = p.n_har;
@@ -2132,9 +2431,14 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr) {
         gq.K = Nth;
         QmomEpi eq{ctx->q, N, nh};
         k_tile_gemm<<<dim3((N + 15) / 16, (nh + 15) / 16, p.n_inst), WG, 0, st>>>(gq, eq);
+        HIPCHK(ctx, mk(0));
         const int cs = (N + WG - 1) / WG;  // N <= 4096 (ggb_create): cs <= 16
         auto har = cs <= 2 ? k_hartree<2> : cs <= 4 ? k_hartree<4> : cs <= 8 ? k_hartree<8> : k_hartree<16>;
         har<<<dim3(nh, p.n_inst), WG, 0, st>>>(ctx->q, ctx->hpow, ctx->V, N, nh, ctx->active);
+        HIPCHK(ctx, mk(1));
+    } else {
+        HIPCHK(ctx, mk(0));
+        HIPCHK(ctx, mk(1));
     }
     GemmOperands gv;  // V(i, theta_j) = sum_L V[L][i] PLT[L][j]
     gv.A = ctx->V;
@@ -2164,6 +2468,7 @@ static int launch_first_half(ggb_ctx *ctx, int step, const int *cur = nullptr) {
 // kinetic kernel variant in use: the one set (ggb_set_kinetic_variant /
 // ggb_autotune) or the default (narrow up to 8 columns, else tiled, K chunks of 16)
 static int kin_variant(const ggb_ctx *ctx) {
+    if (ctx->p.precision == 2) return GGB_KIN_SPLIT16;
     if (ctx->kin_variant != GGB_KIN_DEFAULT) return ctx->kin_variant;
     return ctx->C > KIN_NARROW_MAX_C ? GGB_KIN_TILED16 : GGB_KIN_NARROW;
 }
@@ -2198,6 +2503,26 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh, hipEvent_t e0 =
         k_kin_ref<<<ctx->C, WG, 0, ctx->stream>>>(ctx->psi, ctx->phi_ref, ctx->l0, ctx->eps, ctx->ab,
                                                   p.n_r, ctx->C, p.n_orb, p.dt);
     const int var = kin_variant(ctx);
+    if (var == GGB_KIN_SPLIT16) {
+        const int nct = (2 * ctx->C + TN - 1) / TN;
+        const dim3 gf((p.nk_max + TM - 1) / TM, nct, p.n_l), gb((p.n_r + TM - 1) / TM, nct, p.n_l);
+        const float *Qh = reinterpret_cast<const float *>(ctx->Q);
+        float *Yh = reinterpret_cast<float *>(ctx->Y);
+        const double2 *kph = ctx->kph;
+        const int32_t *nkeep = ctx->nkeep;
+        const double *psi_in = ctx->psi;
+        const float *Ql = ctx->Qlo;
+        float *Yl = ctx->Ylo;
+        double *psi_out = ctx->psi;
+        const int N = p.n_r, nkm = p.nk_max, C = ctx->C, no = p.n_orb;
+        if (pass == 0)
+            kin_launch(k_kin_fwd_split, gf, WG, ctx->stream, e0, e1, Qh, Ql, kph, nkeep, psi_in, ref, Yh,
+                       Yl, N, nkm, C, no);
+        else
+            kin_launch(k_kin_bwd_split, gb, WG, ctx->stream, e0, e1, Qh, Ql, nkeep, (const float *)Yh,
+                       (const float *)Yl, ref, psi_out, N, nkm, C, no);
+        return;
+    }
     if (var >= GGB_KIN_TILED16) {
         const int nct = (2 * ctx->C + TN - 1) / TN;
         const dim3 gf((p.nk_max + TM - 1) / TM, nct, p.n_l), gb((p.n_r + TM - 1) / TM, nct, p.n_l);
@@ -2277,9 +2602,10 @@ static void launch_channels(ggb_ctx *ctx, int step, double2 *dst, const int *cur
     k_chan_dip<<<dim3(ctx->p.n_l, ctx->C), 64, 0, ctx->stream>>>(a);
 }
 
-template <typename Mark = hipError_t (*)()>
+template <typename Mark = hipError_t (*)(), typename MarkK = hipError_t (*)(int)>
 static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int *cur = nullptr,
-                              Mark between = no_mark, double2 *chan_dst = nullptr) {
+                              Mark between = no_mark, double2 *chan_dst = nullptr,
+                              MarkK mk = no_mark_k) {
     launch_angular<1>(ctx, step, cur);
     HIPCHK(ctx, hipGetLastError());
     if (chan_dst) launch_channels(ctx, step, chan_dst, cur);
@@ -2292,7 +2618,7 @@ static int launch_second_half(ggb_ctx *ctx, int step, double *obs_dst, const int
     ob.cur = cur;
     ob.origin = cur ? cur + 1 : nullptr;
     ob.local = step;
-    return launch_potential(ctx, &ob);
+    return launch_potential(ctx, &ob, mk);
 }
 
 extern "C" int ggb_upload_orbitals(ggb_ctx *ctx, const double *psi) {
@@ -2532,8 +2858,9 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         if (!rc)
             rc = launch_second_half(ctx, step, ctx->obs + (size_t)s * ctx->C * 3, nullptr,
                                     [&]() { return mark(s, 4); },
-                                    ctx->channels ? ctx->chan + (size_t)s * ctx->C * ctx->p.n_l : nullptr);
-        HIPCHK(ctx, mark(s, 5));
+                                    ctx->channels ? ctx->chan + (size_t)s * ctx->C * ctx->p.n_l : nullptr,
+                                    [&](int k) { return mark(s, 5 + k); });
+        HIPCHK(ctx, mark(s, 7));
         if (rc) return rc;
     }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -2872,8 +3199,11 @@ extern "C" int ggb_spectrum_noalloc(int32_t device, const double *signal, int32_
 
 extern "C" int ggb_set_kinetic_variant(ggb_ctx *ctx, int32_t variant) {
     if (!ctx) return GGB_E_ARG;
-    if (variant < GGB_KIN_DEFAULT || variant > GGB_KIN_TILED32 ||
-        (variant == GGB_KIN_TILED32 && ctx->p.precision == 0))
+    const bool split = ctx->p.precision == 2;
+    if (variant < GGB_KIN_DEFAULT || variant > GGB_KIN_SPLIT16 ||
+        (variant == GGB_KIN_TILED32 && ctx->p.precision == 0) ||
+        (split && variant != GGB_KIN_DEFAULT && variant != GGB_KIN_SPLIT16) ||
+        (!split && variant == GGB_KIN_SPLIT16))
         return fail(ctx, GGB_E_ARG, "ggb_set_kinetic_variant: unknown or unsupported variant");
     if (variant != ctx->kin_variant) drop_graph(ctx);
     ctx->kin_variant = variant;
@@ -2891,14 +3221,15 @@ extern "C" int ggb_autotune(ggb_ctx *ctx, int32_t reps, int32_t *chosen, double 
     double *saved = nullptr;
     HIPCHK(ctx, hipMalloc((void **)&saved, bytes));
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    double t[GGB_KIN_NVARIANT] = {0.0, 0.0, 0.0, 0.0, 0.0};
     int best = GGB_KIN_DEFAULT, rc = GGB_OK;
     const int prev = ctx->kin_variant;
     hipError_t e = hipMemcpyAsync(saved, ctx->psi, bytes, hipMemcpyDeviceToDevice, ctx->stream);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    const int last = ctx->p.precision ? GGB_KIN_TILED32 : GGB_KIN_TILED16;
-    for (int v = GGB_KIN_NARROW; v <= last && e == hipSuccess && rc == GGB_OK; ++v) {
+    const int first = ctx->p.precision == 2 ? GGB_KIN_SPLIT16 : GGB_KIN_NARROW;
+    const int last = ctx->p.precision == 2 ? GGB_KIN_SPLIT16 : ctx->p.precision ? GGB_KIN_TILED32 : GGB_KIN_TILED16;
+    for (int v = first; v <= last && e == hipSuccess && rc == GGB_OK; ++v) {
         ctx->kin_variant = v;
         rc = launch_kinetic(ctx);  // warm-up
         if (rc == GGB_OK) e = hipEventRecord(e0, ctx->stream);
@@ -2922,7 +3253,7 @@ extern "C" int ggb_autotune(ggb_ctx *ctx, int32_t reps, int32_t *chosen, double 
     ctx->kin_variant = (rc == GGB_OK) ? best : prev;
     if (rc == GGB_OK && chosen) *chosen = best;
     if (rc == GGB_OK && ms)
-        for (int v = 0; v < 4; ++v) ms[v] = t[v];
+        for (int v = 0; v < GGB_KIN_NVARIANT; ++v) ms[v] = t[v];
     return rc;
 }
 

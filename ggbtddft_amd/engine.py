@@ -177,10 +177,10 @@ class Propagator:
             self.set_kinetic_variant(name)
             return name, times
         chosen = ctypes.c_int32()
-        ms = np.zeros(4)
+        ms = np.zeros(len(_lib.KIN_VARIANTS))
         self._check(self.lib.ggb_autotune(self.ctx, int(reps), ctypes.byref(chosen), _lib.dptr(ms)),
                     "ggb_autotune")
-        times = {_lib.KIN_VARIANTS[v]: float(ms[v]) for v in range(1, 4) if ms[v] > 0}
+        times = {_lib.KIN_VARIANTS[v]: float(ms[v]) for v in range(1, len(ms)) if ms[v] > 0}
         name = _lib.KIN_VARIANTS[chosen.value]
         if cache:
             _TUNED[key] = (name, times)

@@ -32,7 +32,10 @@ def grid_for(pulse):
     raise AssertionError
 
 
-def scan_requests(n, seed=0, dt=0.05):
+def scan_requests(n, seed=0, dt=0.05, cycle_scale=1.0):
+    """The draw above; cycle_scale < 1 shortens every pulse by that factor
+    (same grids and fields' shapes, proportionally uneven step counts) for a
+    bounded timing run."""
     rng = np.random.default_rng(seed)
     reqs = []
     for k in range(n):
@@ -41,6 +44,6 @@ def scan_requests(n, seed=0, dt=0.05):
         lam = float(rng.uniform(600.0, 2000.0))
         cyc = int(rng.integers(4, 17))
         nr, lmax, rmax = grid_for(Pulse(I, lam, cyc))
-        reqs.append(Request(rid=k, atom=atom, intensity=I, wavelength=lam, n_cycles=cyc, N=nr,
-                            r_max=rmax, lmax=lmax, dt=dt))
+        reqs.append(Request(rid=k, atom=atom, intensity=I, wavelength=lam,
+                            n_cycles=cyc * float(cycle_scale), N=nr, r_max=rmax, lmax=lmax, dt=dt))
     return reqs

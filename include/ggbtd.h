@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define GGB_ABI_VERSION 3
+#define GGB_ABI_VERSION 4
 
 #define GGB_OK 0
 #define GGB_E_ARG (-1)      /* invalid argument / size */
@@ -58,7 +58,8 @@ typedef struct ggb_problem {
     int32_t precision;    /* 0: FP64 everywhere; 1: kinetic tables, transforms and
                              v_xc in FP32 on the FP32 matrix cores with FP32
                              accumulation; state, Hartree scans and observable
-                             sums stay FP64 */
+                             sums stay FP64; 2: as 1, with the compensated
+                             split-FP32 kinetic product (GGB_KIN_SPLIT16 only) */
     int32_t device;       /* HIP device ordinal */
     double dt;            /* time step (a.u.) */
 } ggb_problem;
@@ -201,6 +202,10 @@ int ggb_set_graph(ggb_ctx *ctx, int32_t steps);
 #define GGB_KIN_NARROW 1
 #define GGB_KIN_TILED16 2
 #define GGB_KIN_TILED32 3
+#define GGB_KIN_SPLIT16 4   /* precision 2 only: TILED16 tiles, operands split into
+                               FP32 hi + lo parts (a_hi b_hi + a_hi b_lo + a_lo b_hi,
+                               three FP32 matrix-core products), FP64 chunk sums */
+#define GGB_KIN_NVARIANT 5
 
 /* Use kinetic variant `variant` (GGB_KIN_*) from now on.  No counterpart in
  * the reference (one cuBLAS batched GEMM). */
@@ -209,15 +214,17 @@ int ggb_set_kinetic_variant(ggb_ctx *ctx, int32_t variant);
 /* Autotuner: time both kinetic passes of every variant valid for this
  * context (reps launches each, HIP events, on the context's own tables and
  * orbitals, which are restored afterwards), keep the fastest, and return it
- * in *chosen and the mean ms per variant in ms[4] (0 for variants not
+ * in *chosen and the mean ms per variant in ms[GGB_KIN_NVARIANT] (0 for variants not
  * tried; ms may be NULL).  Needs ggb_upload_static and ggb_upload_orbitals.
  * No counterpart in the reference. */
 int ggb_autotune(ggb_ctx *ctx, int32_t reps, int32_t *chosen, double *ms);
 
 /* Step phases timed by ggb_set_timing: 0 = half potential + laser,
  * 1 = kinetic pass Q^T, 2 = kinetic pass Q, 3 = laser + half potential +
- * observables, 4 = potential rebuild (density, v_xc, Hartree, phases). */
-#define GGB_NPHASE 5
+ * density tiles + observable partials, 4 = density, v_xc and multipole
+ * moments (+ observable sums), 5 = Hartree scans, 6 = dV and the phase
+ * factors of the next step. */
+#define GGB_NPHASE 7
 
 /* Time the step phases of later ggb_run calls with HIP events on the
  * context stream and reset the accumulated times.  Mode 1: the product

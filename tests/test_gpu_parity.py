@@ -259,7 +259,7 @@ def test_timing_modes(precision, n_inst):
         if mode == 1:
             assert tim["kinetic_qt"][1] == 2 and tim["kinetic_q"][1] == 2
             assert tim["kinetic_qt"][0] > 0.0 and tim["kinetic_q"][0] > 0.0
-            assert tim["potential_rebuild"][1] == 0
+            assert all(tim[k][1] == 0 for k in ("density_moments", "hartree", "dv_phase"))
         if mode == 2:
             assert all(v[1] == 32 and v[0] > 0.0 for v in tim.values())
     for o in outs[1:]:
@@ -319,3 +319,32 @@ def test_channel_dipoles_match_oracle():
         got = prop.run(0, 21, channels=True)
     assert np.array_equal(got["chan"], out["chan"])
     assert np.array_equal(got["dip"], out["dip"])
+
+
+def test_compensated_split_kinetic_product():
+    """The kinetic sub-step in the FP32 mixed form against the FP64 oracle on
+    a perturbed ground state: plain FP32 tiles (precision 1) and the
+    compensated split-FP32 product (precision 2, hi/lo operands, FP64 chunk
+    sums).  The split product removes the rounding of the operands; what is
+    left is the FP32 rounding of each matrix-core result inside a 16-value
+    chunk, which no split of the operands reaches.  Measured: 2.25e-8 plain,
+    1.65e-8 split (of max |psi|)."""
+    _say("case: kinetic sub-step, plain FP32 tiles vs split-FP32 product, Ar N=48 Lmax=6")
+    dt = 0.05
+    S, O = _pair("Ar", 48, 40.0, 6, dt)
+    psi = _random_state(S)
+    ref = psi.copy()
+    O.kinetic(ref)
+    scale = np.abs(ref).max()
+    errs = {}
+    for precision, variant in ((1, "tiled16"), (2, "split16")):
+        with _engine().Propagator(S, precision=precision) as prop:
+            prop.set_field(np.zeros(2))
+            prop.set_kinetic_variant(variant)
+            prop.upload_orbitals(psi)
+            prop.substep(1, step=0)
+            got = prop.download_orbitals()
+        errs[variant] = np.abs(got - ref).max() / scale
+        _say(variant, "max error / max |psi|", errs[variant])
+    assert errs["tiled16"] < 1e-7
+    assert errs["split16"] < 0.85 * errs["tiled16"]
