@@ -309,18 +309,24 @@ def scan_run(n, cycle_scale=1.0):
                        "(N 256/384/512, L_max 32/48/64), dt 0.05, all requests at t = 0"
                        % ("" if cycle_scale == 1.0 else " scaled by %g" % cycle_scale)}
     res = {}
+    from ggbtddft_amd.scan import ScanGroup
     for prec, name in ((0, "fp64"), (1, "fp32")):
-        log("scan %s: serving %d requests on %d lanes" % (name, n, len(keys)))
-        srv = Server(slots=slots, chunk=256, ring=1 << 17, precision=prec)
-        srv.systems.update(systems)
-        results, stats = srv.serve(reqs, realtime=False)
-        srv.close()
+        log("scan %s: %d requests on %d lanes, one graph launch per chunk" % (name, n, len(keys)))
+        results, stats = ScanGroup(systems, slots=slots, chunk=256, ring=1 << 17,
+                                   precision=prec).run(reqs)
         res[name] = results
         out[name] = {"wall_s": stats["wall_s"], "served": stats["served"],
                      "orbital_timesteps_per_s": stats["orbital_timesteps_per_s"],
-                     "device_bytes": stats["device_bytes"]}
+                     "driver": "scan.ScanGroup: all lanes' chunks in one hipGraph launch"}
         log("scan %s: %d served in %.1f s, %.0f orbital-timesteps/s" % (
             name, stats["served"], stats["wall_s"], stats["orbital_timesteps_per_s"]))
+    # the same scan through the serving loop (one host thread per lane)
+    srv = Server(slots=slots, chunk=256, ring=1 << 17, precision=1)
+    srv.systems.update(systems)
+    _, stats = srv.serve(reqs, realtime=False)
+    srv.close()
+    out["fp32_serving_threads"] = {"wall_s": stats["wall_s"], "served": stats["served"],
+                                   "orbital_timesteps_per_s": stats["orbital_timesteps_per_s"]}
     err = []
     for a, b in zip(res["fp64"], res["fp32"]):
         if a.error is None and b.error is None:

@@ -21,7 +21,8 @@ EXPORTS = ("ggb_abi_version", "ggb_create", "ggb_upload_static", "ggb_upload_orb
            "ggb_field_ring", "ggb_set_field_slice", "ggb_upload_columns", "ggb_set_active",
            "ggb_rebuild_potential", "ggb_footprint", "ggb_selftest_copy", "ggb_set_graph",
            "ggb_spectrum", "ggb_set_kinetic_variant", "ggb_autotune", "ggb_ground_state",
-           "ggb_set_channels", "ggb_get_channels", "ggb_spectrum_noalloc", "ggb_last_error",
+           "ggb_set_channels", "ggb_get_channels", "ggb_spectrum_noalloc", "ggb_group_create",
+           "ggb_group_run", "ggb_group_last_error", "ggb_group_destroy", "ggb_last_error",
            "ggb_destroy")
 KIN_VARIANTS = ("default", "narrow", "tiled16", "tiled32", "split16")
 NPHASE = 7
@@ -131,6 +132,16 @@ def load():
     lib.ggb_set_graph.argtypes = [vp, ctypes.c_int32]
     lib.ggb_selftest_mfma.restype = ctypes.c_int
     lib.ggb_selftest_mfma.argtypes = [ctypes.c_int32, _dp]
+    _pp = ctypes.POINTER(_dp)
+    lib.ggb_group_create.restype = ctypes.c_int
+    lib.ggb_group_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.POINTER(vp)]
+    lib.ggb_group_run.restype = ctypes.c_int
+    lib.ggb_group_run.argtypes = [vp, _ip, ctypes.c_int32, _pp, _pp, _pp]
+    lib.ggb_group_last_error.restype = ctypes.c_char_p
+    lib.ggb_group_last_error.argtypes = [vp]
+    lib.ggb_group_destroy.restype = None
+    lib.ggb_group_destroy.argtypes = [vp]
     lib.ggb_last_error.restype = ctypes.c_char_p
     lib.ggb_last_error.argtypes = [vp]
     lib.ggb_destroy.restype = None

@@ -1933,6 +1933,7 @@ struct ggb_ctx {
     int graph_steps;
     hipGraphExec_t graph_exec;
     int *dcur;  // [0] = step counter of the replay, [1] = first step of the run call
+    unsigned long gen;  // incremented whenever a captured step may no longer be valid
     // optional HIP-event timing of the step phases (ggb_set_timing)
     int timing;                     // 0 off, 1 kinetic only, 2 every phase
     std::vector<hipEvent_t> events; // GGB_NPHASE + 1 per step of a run call
@@ -1958,6 +1959,7 @@ static int fail(ggb_ctx *ctx, int code, const std::string &msg) {
 
 // a captured graph holds raw buffer pointers: drop it whenever one may move
 static void drop_graph(ggb_ctx *ctx) {
+    ++ctx->gen;  // graphs of groups holding this context are captured again too
     if (ctx->graph_exec) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -2080,6 +2082,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ctx->graph_steps = 0;
     ctx->graph_exec = nullptr;
     ctx->dcur = nullptr;
+    ctx->gen = 0;
     for (int k = 0; k < GGB_NPHASE; ++k) {
         ctx->phase_ms[k] = 0.0;
         ctx->phase_n[k] = 0;
@@ -2893,6 +2896,239 @@ extern "C" int ggb_run(ggb_ctx *ctx, int32_t step0, int32_t nsteps, double *dip,
         }
         delete[] h;
     }
+    return GGB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Groups: the step chunks of several contexts (e.g. the lanes of a batched
+// scan, one context per grid) replayed by one hipGraph launch.  The graph is
+// captured on the group's own stream: it forks to every context's stream, runs
+// graph_steps steps of each context there (field index and observable row
+// from each context's device step counter, as in run_graph), and joins back;
+// contexts therefore overlap on the device with one host launch per chunk.
+// The graph is captured again whenever one of its contexts changed anything a
+// captured step holds (a drop of that context's own graph bumps ctx->gen).
+// ---------------------------------------------------------------------------
+struct ggb_group {
+    std::vector<ggb_ctx *> ctx;
+    std::vector<unsigned long> gen;  // generations at capture
+    int steps;
+    int device;
+    hipStream_t lead;
+    hipEvent_t fork;
+    std::vector<hipEvent_t> join, ready;
+    hipGraphExec_t exec;
+    size_t obs_rows;  // rows of every context's observable buffer when captured
+    std::string err;
+};
+
+static int gfail(ggb_group *g, int code, const std::string &m) {
+    g->err = m;
+    return code;
+}
+
+#define GHIP(g, call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) return gfail((g), GGB_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" void ggb_group_destroy(ggb_group *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->lead) (void)hipStreamSynchronize(g->lead);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    for (hipEvent_t e : g->join) (void)hipEventDestroy(e);
+    for (hipEvent_t e : g->ready) (void)hipEventDestroy(e);
+    if (g->lead) (void)hipStreamDestroy(g->lead);
+    delete g;
+}
+
+extern "C" int ggb_group_create(ggb_ctx *const *ctxs, int32_t n, int32_t steps, ggb_group **out) {
+    if (!out) return GGB_E_ARG;
+    *out = nullptr;
+    if (!ctxs || n < 1 || n > 64 || steps < 1 || steps > 4096) {
+        g_create_err = "ggb_group_create: 1..64 contexts and 1..4096 steps per graph";
+        return GGB_E_ARG;
+    }
+    for (int k = 0; k < n; ++k) {
+        if (!ctxs[k] || ctxs[k]->p.device != ctxs[0]->p.device) {
+            g_create_err = "ggb_group_create: null context or contexts on different devices";
+            return GGB_E_ARG;
+        }
+        for (int j = 0; j < k; ++j)
+            if (ctxs[j] == ctxs[k]) {
+                g_create_err = "ggb_group_create: a context appears twice";
+                return GGB_E_ARG;
+            }
+    }
+    ggb_group *g = new ggb_group();
+    g->ctx.assign(ctxs, ctxs + n);
+    g->gen.assign(n, 0);
+    g->steps = steps;
+    g->device = ctxs[0]->p.device;
+    g->lead = nullptr;
+    g->fork = nullptr;
+    g->exec = nullptr;
+    g->obs_rows = 0;
+    bool ok = hipSetDevice(g->device) == hipSuccess &&
+              hipStreamCreateWithFlags(&g->lead, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < n && ok; ++k) {
+        hipEvent_t a = nullptr, b = nullptr;
+        ok = hipEventCreateWithFlags(&a, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&b, hipEventDisableTiming) == hipSuccess;
+        if (a) g->join.push_back(a);
+        if (b) g->ready.push_back(b);
+    }
+    if (!ok) {
+        g_create_err = "ggb_group_create: stream or event creation failed";
+        ggb_group_destroy(g);
+        return GGB_E_HIP;
+    }
+    *out = g;
+    return GGB_OK;
+}
+
+extern "C" const char *ggb_group_last_error(const ggb_group *g) {
+    return g ? g->err.c_str() : g_create_err.c_str();
+}
+
+static int group_capture(ggb_group *g) {
+    if (g->exec) {
+        (void)hipGraphExecDestroy(g->exec);
+        g->exec = nullptr;
+    }
+    const int G = g->steps;
+    hipGraph_t graph;
+    GHIP(g, hipStreamBeginCapture(g->lead, hipStreamCaptureModeThreadLocal));
+    int rc = GGB_OK;
+    hipError_t e = hipEventRecord(g->fork, g->lead);
+    for (size_t k = 0; k < g->ctx.size() && e == hipSuccess && !rc; ++k) {
+        ggb_ctx *c = g->ctx[k];
+        e = hipStreamWaitEvent(c->stream, g->fork, 0);
+        for (int s = 0; s < G && e == hipSuccess && !rc; ++s) {
+            rc = launch_first_half(c, s, c->dcur);
+            if (!rc) rc = launch_kinetic(c, true);
+            if (!rc) rc = launch_second_half(c, s, c->obs, c->dcur, no_mark, c->channels ? c->chan : nullptr);
+        }
+        if (!rc && e == hipSuccess) {
+            k_bump<<<1, 64, 0, c->stream>>>(c->dcur, G);
+            e = hipEventRecord(g->join[k], c->stream);
+        }
+        if (e == hipSuccess) e = hipStreamWaitEvent(g->lead, g->join[k], 0);
+        if (rc) e = hipErrorUnknown;
+        if (rc) g->err = "group capture: " + c->err;
+    }
+    hipError_t ee = hipStreamEndCapture(g->lead, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return gfail(g, GGB_E_HIP, std::string("group capture: ") + hipGetErrorString(e));
+    if (ee != hipSuccess) return gfail(g, GGB_E_HIP, std::string("group capture end: ") + hipGetErrorString(ee));
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g->exec = nullptr;
+        return gfail(g, GGB_E_HIP, std::string("group graph instantiate: ") + hipGetErrorString(e));
+    }
+    for (size_t k = 0; k < g->ctx.size(); ++k) g->gen[k] = g->ctx[k]->gen;
+    return GGB_OK;
+}
+
+extern "C" int ggb_group_run(ggb_group *g, const int32_t *step0, int32_t nsteps, double *const *dip,
+                             double *const *acc, double *const *norm) {
+    if (!g) return GGB_E_ARG;
+    if (!step0 || nsteps < 0) return gfail(g, GGB_E_ARG, "ggb_group_run: null step0 or negative steps");
+    if (nsteps == 0) return GGB_OK;
+    const size_t n = g->ctx.size();
+    for (size_t k = 0; k < n; ++k) {
+        ggb_ctx *c = g->ctx[k];
+        if (!c->have_orbitals) return gfail(g, GGB_E_STATE, "ggb_group_run: a context has no orbitals");
+        if (c->timing) return gfail(g, GGB_E_STATE, "ggb_group_run: timing is on in a context");
+        if (step0[k] < 0 || (!c->field_ring && (long)step0[k] + nsteps > c->n_field))
+            return gfail(g, GGB_E_ARG, "ggb_group_run: steps outside a context's field");
+    }
+    GHIP(g, hipSetDevice(g->device));
+    // buffers the captured steps write: observables (and channels) for nsteps rows
+    bool moved = false;
+    for (size_t k = 0; k < n; ++k) {
+        ggb_ctx *c = g->ctx[k];
+        const size_t need = (size_t)nsteps * c->C * 3;
+        if (need > c->obs_cap) {
+            drop_graph(c);
+            GHIP(g, hipStreamSynchronize(c->stream));
+            if (c->obs) (void)hipFree(c->obs);
+            c->obs = nullptr;
+            c->obs_cap = 0;
+            GHIP(g, dalloc(&c->obs, need));
+            c->obs_cap = need;
+            moved = true;
+        }
+        const size_t cneed = c->channels ? (size_t)nsteps * c->C * c->p.n_l : 0;
+        if (cneed > c->chan_cap) {
+            drop_graph(c);
+            GHIP(g, hipStreamSynchronize(c->stream));
+            if (c->chan) (void)hipFree(c->chan);
+            c->chan = nullptr;
+            c->chan_cap = 0;
+            GHIP(g, dalloc(&c->chan, cneed));
+            c->chan_cap = cneed;
+            moved = true;
+        }
+        c->chan_rows = c->channels ? nsteps : 0;
+        if (!c->dcur) {
+            GHIP(g, hipMalloc((void **)&c->dcur, 2 * sizeof(int)));
+            moved = true;
+        }
+    }
+    const int G = g->steps;
+    const int reps = nsteps / G;
+    bool stale = moved || !g->exec;
+    for (size_t k = 0; k < n && !stale; ++k) stale = g->gen[k] != g->ctx[k]->gen;
+    if (reps > 0 && stale) {
+        for (size_t k = 0; k < n; ++k) GHIP(g, hipStreamSynchronize(g->ctx[k]->stream));
+        int rc = group_capture(g);
+        if (rc) return rc;
+    }
+    // step counters, then the graph launches on the lead stream after them
+    for (size_t k = 0; k < n; ++k) {
+        ggb_ctx *c = g->ctx[k];
+        int h[2] = {step0[k], step0[k]};
+        GHIP(g, hipMemcpyAsync(c->dcur, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+        GHIP(g, hipEventRecord(g->ready[k], c->stream));
+        GHIP(g, hipStreamWaitEvent(g->lead, g->ready[k], 0));
+    }
+    for (int r = 0; r < reps; ++r) GHIP(g, hipGraphLaunch(g->exec, g->lead));
+    GHIP(g, hipEventRecord(g->fork, g->lead));
+    // the steps after the last whole graph: plain launches on each context's stream
+    for (size_t k = 0; k < n; ++k) {
+        ggb_ctx *c = g->ctx[k];
+        GHIP(g, hipStreamWaitEvent(c->stream, g->fork, 0));
+        for (int s = reps * G; s < nsteps; ++s) {
+            const int step = (int)(((long)step0[k] + s) % c->n_field);
+            int rc = launch_first_half(c, step);
+            if (!rc) rc = launch_kinetic(c, true);
+            if (!rc)
+                rc = launch_second_half(c, step, c->obs + (size_t)s * c->C * 3, nullptr, no_mark,
+                                        c->channels ? c->chan + (size_t)s * c->C * c->p.n_l : nullptr);
+            if (rc) return gfail(g, rc, "ggb_group_run: " + c->err);
+        }
+    }
+    for (size_t k = 0; k < n; ++k) {
+        ggb_ctx *c = g->ctx[k];
+        GHIP(g, hipStreamSynchronize(c->stream));
+        if (!(dip && dip[k]) && !(acc && acc[k]) && !(norm && norm[k])) continue;
+        const size_t need = (size_t)nsteps * c->C * 3;
+        std::vector<double> h(need);
+        GHIP(g, hipMemcpyAsync(h.data(), c->obs, need * 8, hipMemcpyDeviceToHost, c->stream));
+        GHIP(g, hipStreamSynchronize(c->stream));
+        for (size_t q = 0; q < (size_t)nsteps * c->C; ++q) {
+            if (dip && dip[k]) dip[k][q] = h[3 * q + 0];
+            if (acc && acc[k]) acc[k][q] = h[3 * q + 1];
+            if (norm && norm[k]) norm[k][q] = h[3 * q + 2];
+        }
+    }
+    GHIP(g, hipStreamSynchronize(g->lead));
     return GGB_OK;
 }
 

@@ -224,6 +224,54 @@ class Propagator:
         self.close()
 
 
+class Group:
+    """Several propagators (e.g. one per grid of a batched scan) advanced
+    together: whole runs of `steps` steps of every member come from one
+    hipGraph launch (ggb_group_run), each member on its own stream."""
+
+    def __init__(self, props, steps=64):
+        self.lib = _lib.load()
+        self.props = list(props)
+        arr = (ctypes.c_void_p * len(self.props))(*[p.ctx.value for p in self.props])
+        g = ctypes.c_void_p()
+        rc = self.lib.ggb_group_create(arr, len(self.props), int(steps), ctypes.byref(g))
+        if rc != 0:
+            raise _lib.GGBError("ggb_group_create failed: %s"
+                                % self.lib.ggb_group_last_error(None).decode())
+        self.g = g
+
+    def run(self, step0, nsteps):
+        """Advance member k from step0[k] by nsteps; returns one dict(dip, acc,
+        norm) per member, [nsteps, n_inst, n_orb] each."""
+        n = len(self.props)
+        s0 = np.ascontiguousarray(np.asarray(step0, dtype=np.int32))
+        assert s0.shape == (n,)
+        outs = [dict(dip=np.empty((nsteps, p.C)), acc=np.empty((nsteps, p.C)),
+                     norm=np.empty((nsteps, p.C))) for p in self.props]
+        ptrs = {k: (_lib._dp * n)(*[_lib.dptr(o[k]) for o in outs]) for k in ("dip", "acc", "norm")}
+        rc = self.lib.ggb_group_run(self.g, _lib.iptr(s0), int(nsteps), ptrs["dip"], ptrs["acc"],
+                                    ptrs["norm"])
+        if rc != 0:
+            raise _lib.GGBError("ggb_group_run failed (%s): %s"
+                                % (_lib.ERRORS.get(rc, rc), self.lib.ggb_group_last_error(self.g).decode()))
+        for p, o in zip(self.props, outs):
+            sh = (nsteps, p.n_inst, p.sys.n_orb)
+            for k in o:
+                o[k] = o[k].reshape(sh)
+        return outs
+
+    def close(self):
+        if getattr(self, "g", None):
+            self.lib.ggb_group_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def selftest_mfma(device=0):
     """[2, 16, 16]: the FP64 and FP32 16x16x4 matrix-core products."""
     lib = _lib.load()

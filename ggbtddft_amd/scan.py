@@ -15,10 +15,14 @@ radius alpha = E0 / omega0^2: N = 256, Lmax = 32, r_max = 100 below 25 a.u.,
 N = 384, Lmax = 48, r_max = 150 below 60 a.u., else N = 512, Lmax = 64,
 r_max = 200; dt = 0.05.
 """
+import math
+import time
+
 import numpy as np
 
+from . import spectrum as spec
 from .pulse import Pulse
-from .serving import Request
+from .serving import Lane, Request, Result
 
 GRIDS = ((25.0, 256, 32, 100.0), (60.0, 384, 48, 150.0), (np.inf, 512, 64, 200.0))
 
@@ -47,3 +51,88 @@ def scan_requests(n, seed=0, dt=0.05, cycle_scale=1.0):
         reqs.append(Request(rid=k, atom=atom, intensity=I, wavelength=lam,
                             n_cycles=cyc * float(cycle_scale), N=nr, r_max=rmax, lmax=lmax, dt=dt))
     return reqs
+
+
+class ScanGroup:
+    """A batched scan served as one batch on one GPU: one lane (context) per
+    static key with a slot per request of that key, the chunks of all busy
+    lanes advanced by one hipGraph launch (engine.Group), requests seated in
+    free slots at chunk boundaries until all have run their own number of
+    steps.  No host threads: one loop launches every lane's chunk at once.
+    Results are each request's observables, Ehrenfest acceleration and
+    spectrum (as the serving loop's)."""
+
+    def __init__(self, systems, slots, chunk=256, ring=1 << 17, precision=0, device=0,
+                 graph_steps=64, spectrum_beta=24.0):
+        import functools
+        from . import engine
+        self.systems, self.slots, self.chunk, self.ring = systems, slots, chunk, ring
+        self.device, self.graph_steps, self.beta = device, graph_steps, spectrum_beta
+        self.factory = functools.partial(engine.Propagator, precision=int(precision))
+        self.engine = engine
+
+    def run(self, requests):
+        """Serve every request (arrival times ignored); returns (results in
+        request order, stats)."""
+        t0 = time.perf_counter()
+        waiting = {}
+        for r in requests:
+            waiting.setdefault(r.key(), []).append(r)
+        lanes = {k: Lane(self.systems[k], min(self.slots, len(v)), self.chunk, self.ring,
+                         self.factory, self.device) for k, v in waiting.items()}
+        results, errors = {}, {}
+        group, members = None, None
+        try:
+            while len(results) + len(errors) < len(requests):
+                for k, lane in lanes.items():
+                    q = waiting[k]
+                    for slot in lane.free_slots():
+                        if not q:
+                            break
+                        req = q.pop(0)
+                        try:
+                            lane.seat(req, slot, time.perf_counter() - t0)
+                        except Exception as exc:  # a bad request fails alone
+                            errors[req.rid] = repr(exc)
+                busy = [k for k, lane in lanes.items() if lane.busy()]
+                if not busy:
+                    if not any(waiting.values()):
+                        break  # every request has failed or finished
+                    continue
+                if busy != members:  # the group follows the lanes with work
+                    if group is not None:
+                        group.close()
+                    group = self.engine.Group([lanes[k].prop for k in busy], self.graph_steps)
+                    members = busy
+                obs = group.run([lanes[k].step for k in busy], self.chunk)
+                for k, o in zip(busy, obs):
+                    lane = lanes[k]
+                    for a in lane.consume(o):
+                        if a.error is not None:
+                            errors[a.req.rid] = a.error
+                            continue
+                        results[a.req.rid] = self._finish(lane, a, t0)
+        finally:
+            if group is not None:
+                group.close()
+            for lane in lanes.values():
+                lane.prop.close()
+        out = [results.get(r.rid) or Result(request=r, error=errors.get(r.rid, "not served"))
+               for r in requests]
+        wall = time.perf_counter() - t0
+        ots = sum(l.orbital_steps for l in lanes.values())
+        tts = np.array([r.time_to_spectrum for r in out if r.error is None])
+        return out, dict(wall_s=wall, served=int(tts.size), failed=len(requests) - int(tts.size),
+                         orbital_timesteps_per_s=ots / wall if wall > 0 else math.nan,
+                         lanes=len(lanes))
+
+    def _finish(self, lane, a, t0):
+        d = np.concatenate([b[0] for b in a.buf])
+        ac = np.concatenate([b[1] for b in a.buf])
+        nr = np.concatenate([b[2] for b in a.buf])
+        res = Result(request=a.req, dip=d, acc=ac, norm=nr)
+        res.a_total = lane.system.total_acceleration(ac, nr, a.E_end)
+        res.omega, res.spectrum, _ = spec.device_spectrum(res.a_total, a.req.dt, beta=self.beta,
+                                                          device=self.device)
+        res.t_finish = time.perf_counter() - t0
+        return res

@@ -245,8 +245,12 @@ class Lane:
     def advance(self):
         """Run one chunk; return the requests that ended in it (finished, or
         stopped with ``error`` set: observables not finite or norm growth)."""
+        return self.consume(self.prop.run(self.step, self.chunk))
+
+    def consume(self, obs):
+        """Take the observables of the chunk just run (from self.step): hand
+        each request its rows, and return the requests that ended in it."""
         k = self.chunk
-        obs = self.prop.run(self.step, k)
         self.step += k
         self.last_used = time.perf_counter()
         ended = []

@@ -193,6 +193,34 @@ int ggb_debug_substep(ggb_ctx *ctx, int32_t which, int32_t step);
  * a capture in another thread.  No counterpart in the reference. */
 int ggb_set_graph(ggb_ctx *ctx, int32_t steps);
 
+/* Groups of contexts replayed together: the chunks of several contexts (the
+ * lanes of a batched scan, one context per grid; every context on the same
+ * device, each used by no other thread meanwhile) run from one hipGraph
+ * launch per `steps` steps, each context on its own stream inside the graph.
+ * Counterpart of the reference's sequential scan loop (main.cu:460-481),
+ * which runs one scan after another. */
+typedef struct ggb_group ggb_group;
+
+/* Create a group of n contexts (1..64, distinct) replayed `steps` steps
+ * (1..4096) per graph launch.  Errors: ggb_group_last_error(NULL). */
+int ggb_group_create(ggb_ctx *const *ctxs, int32_t n, int32_t steps, ggb_group **out);
+
+/* Propagate every context of the group by nsteps steps from its own step0[k]
+ * (field index as in ggb_run) and copy each context's observables into
+ * dip[k]/acc[k]/norm[k] ([nsteps][n_inst*n_orb] each; any array or entry may
+ * be NULL).  Whole graphs of `steps` steps run from one launch, the rest as
+ * plain launches; results equal ggb_run of each context alone, bit for bit.
+ * The graph is captured on the first call and again after any call that
+ * changes what a context's captured step holds (ggb_set_field,
+ * ggb_set_kinetic_variant, ggb_set_channels, ...). */
+int ggb_group_run(ggb_group *g, const int32_t *step0, int32_t nsteps, double *const *dip,
+                  double *const *acc, double *const *norm);
+
+/* Message of the last failure on g (or of the last failed ggb_group_create). */
+const char *ggb_group_last_error(const ggb_group *g);
+
+void ggb_group_destroy(ggb_group *g);
+
 /* Kinetic-pass kernel variants (tile shapes).  DEFAULT: NARROW up to 8
  * orbital columns, else TILED16.  NARROW: one 16x16 tile per work-group with
  * the contraction split over its four waves; TILED16 / TILED32: 64x128
