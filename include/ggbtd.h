@@ -276,8 +276,6 @@ int ggb_selftest_mfma(int32_t device, double *out);
  * kernel against which the profiler's HBM byte counters are calibrated. */
 int ggb_selftest_copy(int32_t device, uint64_t bytes, double *ms);
 
-/* Message of the last failure on ctx (or of the last failed ggb_create when
- * ctx is NULL). */
 /* HHG spectrum of one time series on the device: Kaiser window (beta,
  * sym = False) on signal[n], the 8th-order central second difference when
  * `differentiate` (a dipole; else an acceleration), zero padding by
@@ -337,6 +335,8 @@ int ggb_ground_state(const ggb_gs_problem *p, const int32_t *shell_l, const doub
                      const double *lam0, double *u, double *eps, double *v_gs, double *rho_u,
                      int32_t *iterations, int32_t *converged);
 
+/* Message of the last failure on ctx (or of the last failed ggb_create when
+ * ctx is NULL). */
 const char *ggb_last_error(const ggb_ctx *ctx);
 
 void ggb_destroy(ggb_ctx *ctx);
