@@ -1147,11 +1147,35 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         // barrier (rows l < m and l >= N_l back to zero)
         T svr[NT][4], dvr[NT][4];
         if (wave < mt) {
+            // phase factors at this lane's theta rows j and j' = Nth - 1 - j of
+            // each column tile, loaded ahead of the forward products that cover
+            // their latency: ph[u][r] = (f_j.x, f_j.y, f_j'.x, f_j'.y)
+            T ph[NT][4][4];
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const size_t pbase = ((size_t)inst * g.N + (oku[u] ? iu[u] : 0)) * g.Nth;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = wave * 16 + mfma_t<T>::row(kr, r);
+                    const int jq = j < g.h ? j : 0;  // rows past the half: any valid row, unused
+                    const int jm = g.Nth - 1 - jq;
+                    if (mfma_t<T>::mixed) {
+                        // mixed form: the table holds (f - 1), formed in FP64 before the rounding
+                        const float2 *pf = reinterpret_cast<const float2 *>(g.phase) + pbase;
+                        const float2 fa_ = pf[jq], fb_ = pf[jm];
+                        ph[u][r][0] = fa_.x, ph[u][r][1] = fa_.y, ph[u][r][2] = fb_.x, ph[u][r][3] = fb_.y;
+                    } else {
+                        const double2 *pf = reinterpret_cast<const double2 *>(g.phase) + pbase;
+                        const double2 fa_ = pf[jq], fb_ = pf[jm];
+                        ph[u][r][0] = (T)fa_.x, ph[u][r][1] = (T)fa_.y;
+                        ph[u][r][2] = (T)fb_.x, ph[u][r][3] = (T)fb_.y;
+                    }
+                }
+            }
             v4 e[NT], o[NT];
             ang_fwd<T, NT>(Bs, Ba, X, g, pr, m, wave, e, o);
 #pragma unroll
             for (int u = 0; u < NT; ++u) {
-                const size_t pbase = ((size_t)inst * g.N + (oku[u] ? iu[u] : 0)) * g.Nth;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int j = wave * 16 + mfma_t<T>::row(kr, r);
@@ -1160,22 +1184,11 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                     const T pao = __shfl_xor(pa, 1), pbo = __shfl_xor(pb, 1);
                     T sv = (T)0, dv = (T)0;
                     if (j < g.h && oku[u]) {
-                        T fax, fay, fbx, fby;
-                        if (mfma_t<T>::mixed) {
-                            // mixed form: the table holds (f - 1), formed in FP64 before the rounding
-                            const float2 *ph = reinterpret_cast<const float2 *>(g.phase) + pbase;
-                            const float2 fa_ = ph[j], fb_ = ph[jm];
-                            fax = fa_.x, fay = fa_.y, fbx = fb_.x, fby = fb_.y;
-                        } else {
-                            const double2 *ph = reinterpret_cast<const double2 *>(g.phase) + pbase;
-                            const double2 fa_ = ph[j], fb_ = ph[jm];
-                            fax = (T)fa_.x, fay = (T)fa_.y, fbx = (T)fb_.x, fby = (T)fb_.y;
-                        }
                         // component p of f (x + i y): x fx - y fy (p = 0), y fx + x fy (p = 1);
                         // this lane holds component p, its partner the other
                         const T sg = p ? (T)1 : (T)-1;
-                        const T qa = pa * fax + sg * pao * fay;
-                        const T qb = pb * fbx + sg * pbo * fby;
+                        const T qa = pa * ph[u][r][0] + sg * pao * ph[u][r][1];
+                        const T qb = pb * ph[u][r][2] + sg * pbo * ph[u][r][3];
                         sv = (jm == j) ? qa : qa + qb;
                         dv = (jm == j) ? qa : qa - qb;
                     }
