@@ -265,16 +265,18 @@ def serving_run(sysm_ar, args):
     out["mixed_stream_warm"] = warm
     out["mixed_stream_cold"] = cold
     if args.serving_saturated > 0:
-        n = args.serving_saturated
-        arr = poisson_arrivals(100.0, n, seed=1)
+        n, slots, rate = args.serving_saturated, args.serving_saturated_slots, 1000.0
+        arr = poisson_arrivals(rate, n, seed=1)
         rng = np.random.default_rng(1)
         sat = [Request(rid=k, atom=ATOM, intensity=1.5e14, wavelength=800.0, n_cycles=2, N=N,
                        r_max=R_MAX, lmax=LMAX, dt=DT, arrival=float(arr[k]),
                        second=(2, 1e-3, float(rng.uniform(0.0, 2.0 * np.pi)))) for k in range(n)]
-        st, _ = _serve(sat, n, args.serving_chunk, systems={(ATOM, N, float(R_MAX), LMAX, float(DT)): sysm_ar})
+        st, _ = _serve(sat, slots, args.serving_chunk,
+                       systems={(ATOM, N, float(R_MAX), LMAX, float(DT)): sysm_ar})
         st["workload"] = ("%d in-situ Ar requests (N=512 Lmax=64 dt=0.02, 800 nm 2 cycles at "
                           "1.5e14 W/cm^2 + 400 nm at 1e-3, random phase), Poisson arrivals at "
-                          "100/s, one lane of %d slots" % (n, n))
+                          "%g/s, one lane of %d slots: the requests beyond the slots wait, so "
+                          "every slot stays busy until the last wave drains" % (n, rate, slots))
         log("serving saturated: %.0f orbital-timesteps/s, p50 %.2f s, p99 %.2f s"
             % (st["orbital_timesteps_per_s"], st["p50_time_to_spectrum_s"],
                st["p99_time_to_spectrum_s"]))
@@ -481,8 +483,10 @@ def main():
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-slots", type=int, default=8)
-    ap.add_argument("--serving-saturated", type=int, default=64,
+    ap.add_argument("--serving-saturated", type=int, default=128,
                     help="in-situ argon requests of the saturated serving run (0 = skip)")
+    ap.add_argument("--serving-saturated-slots", type=int, default=64,
+                    help="slots of the saturated run's lane")
     ap.add_argument("--serving-chunk", type=int, default=256,
                     help="steps per lane between seatings (a multiple of 64 keeps graph replay whole)")
     ap.add_argument("--no-precision-report", action="store_true",

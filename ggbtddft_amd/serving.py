@@ -26,12 +26,14 @@ grid, optional second colour) arrive over time and are served concurrently:
   slot at the next chunk boundary; a lane whose chunk raises fails only the
   requests seated in it.
 * A request finishes when its own number of steps is done; its dipole
-  acceleration is turned into a spectrum on the GPU (``ggb_spectrum``) and the
-  request's arrival -> spectrum time is recorded.
+  acceleration is turned into a spectrum on the GPU by a spectrum thread while
+  its lane runs on (``ggb_spectrum_noalloc``, which allocates nothing and so
+  runs beside the chunks; ``ggb_spectrum`` with the device alone for series
+  beyond its scratch), and the request's arrival -> spectrum time is recorded.
 
-Calls that allocate, free or use the legacy stream (lane creation and
-destruction, a chunk that captures a graph, the device spectrum) take the
-device alone; chunk runs share it (``_Gate``).
+Calls that allocate or free (lane creation and destruction, a chunk that
+captures a graph, the allocating spectrum) take the device alone; chunk runs
+and the allocation-free spectrum share it (``_Gate``).
 
 Every instance's arithmetic is column-local in every kernel, so a request's
 result does not depend on what else shares its lane (tests check this).
@@ -341,21 +343,29 @@ class Server:
             self._cancelled.add(rid)
             self._cv.notify_all()
 
-    def _finish(self, lane, a, t0):
+    def _result(self, lane, a):
+        """Observables of a finished request and its total dipole acceleration."""
         d = np.concatenate([b[0] for b in a.buf])
         ac = np.concatenate([b[1] for b in a.buf])
         nr = np.concatenate([b[2] for b in a.buf])
         res = Result(request=a.req, dip=d, acc=ac, norm=nr)
         res.a_total = lane.system.total_acceleration(ac, nr, a.E_end)
-        if self.device_spectrum:
-            with self._gate.exclusive():  # ggb_spectrum allocates and uses the legacy stream
-                res.omega, res.spectrum, _ = spec.device_spectrum(res.a_total, a.req.dt,
-                                                                  beta=self.beta, device=self.device)
-        else:
-            res.omega, res.spectrum = spec.acceleration_spectrum(res.a_total, a.req.dt,
-                                                                 beta=self.beta)
-        res.t_finish = time.perf_counter() - t0
         return res
+
+    def _spectrum(self, res):
+        dt = res.request.dt
+        if not self.device_spectrum:
+            res.omega, res.spectrum = spec.acceleration_spectrum(res.a_total, dt, beta=self.beta)
+        elif spec.noalloc_fits(res.a_total.size):
+            # no allocation: runs beside the lanes' chunks (their non-blocking
+            # streams do not wait on the legacy stream it uses)
+            with self._gate.shared():
+                res.omega, res.spectrum, _ = spec.device_spectrum(res.a_total, dt, beta=self.beta,
+                                                                  device=self.device, noalloc=True)
+        else:
+            with self._gate.exclusive():  # ggb_spectrum allocates
+                res.omega, res.spectrum, _ = spec.device_spectrum(res.a_total, dt, beta=self.beta,
+                                                                  device=self.device)
 
     def _close_lane(self, key):
         lane = self.lanes.pop(key)
@@ -376,6 +386,8 @@ class Server:
         waiting = {}          # key -> deque of arrived requests without a slot
         results, errors, admit_time = {}, {}, {}
         threads = {}          # key -> (thread, stop flag)
+        finished = deque()    # (result without spectrum) for the spectrum thread
+        spec_stop = [False]
         total = len(requests)
         ids = {r.rid for r in requests}
 
@@ -449,14 +461,34 @@ class Server:
                         fail(a.req.rid, a.error)
                         continue
                     try:
-                        res = self._finish(lane, a, t0)
-                    except Exception as exc:  # e.g. the spectrum: this request fails alone
+                        res = self._result(lane, a)
+                    except Exception as exc:  # this request fails alone
                         fail(a.req.rid, repr(exc))
                         continue
-                    with cv:
-                        res.t_admit = admit_time.get(a.req.rid, math.nan)
-                        results[a.req.rid] = res
+                    with cv:  # its spectrum is the spectrum thread's; the lane runs on
+                        finished.append(res)
                         cv.notify_all()
+
+        def spectra():
+            """Spectra of finished requests, one after another, beside the lanes."""
+            while True:
+                with cv:
+                    while not finished and not spec_stop[0]:
+                        cv.wait(0.05)
+                    if not finished:
+                        return
+                    res = finished.popleft()
+                rid = res.request.rid
+                try:
+                    self._spectrum(res)
+                except Exception as exc:  # this request fails alone
+                    fail(rid, repr(exc))
+                    continue
+                res.t_finish = now()
+                with cv:
+                    res.t_admit = admit_time.get(rid, math.nan)
+                    results[rid] = res
+                    cv.notify_all()
 
         def start_lane(key):
             """Create the lane of `key` if its footprint fits (closing idle
@@ -489,6 +521,8 @@ class Server:
             th.start()
             return True
 
+        spec_thread = threading.Thread(target=spectra, daemon=True)
+        spec_thread.start()
         try:
             while True:
                 with cv:
@@ -533,9 +567,11 @@ class Server:
             with cv:
                 for _, stop in threads.values():
                     stop[0] = True
+                spec_stop[0] = True
                 cv.notify_all()
             for th, _ in threads.values():
                 th.join()
+            spec_thread.join()
         out = []
         for req in requests:
             if req.rid in results:

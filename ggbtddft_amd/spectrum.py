@@ -16,6 +16,8 @@ argonInSitu/argonInSitu/arg.py:42-67; ``cutoff_energy`` is 3.17 U_p + I_p
 on the GPU (``ggb_spectrum``: window, difference, Bluestein DFT of any length
 on radix-2 Stockham FFTs, masks), for the serving path's time-to-spectrum.
 """
+import threading
+
 import numpy as np
 from scipy.signal.windows import kaiser
 
@@ -66,11 +68,23 @@ def cutoff_energy(E0, omega0, ip):
     return 3.17 * (E0 / (2.0 * omega0)) ** 2 + abs(ip)
 
 
+NOALLOC_MAX_POINTS = 1 << 19   # (2 n_pad + 1) n of ggb_spectrum_noalloc's fixed scratch
+NOALLOC_MAX_ORDERS = 256
+_noalloc_lock = threading.Lock()  # the fixed scratch is shared by every call on a device
+
+
+def noalloc_fits(n, n_pad=8, n_orders=0):
+    """Whether ggb_spectrum_noalloc takes a series of n samples."""
+    return (2 * n_pad + 1) * n <= NOALLOC_MAX_POINTS and n_orders <= NOALLOC_MAX_ORDERS
+
+
 def device_spectrum(signal, dt, differentiate=False, beta=24.0, n_pad=8, omega0=None,
-                    orders=None, device=0, want_power=True):
+                    orders=None, device=0, want_power=True, noalloc=False):
     """(w, power, intensities) of ``dipole_spectrum`` (differentiate=True) or
     ``acceleration_spectrum`` computed on the GPU; intensities of ``orders``
-    with the masks of ``harmonic_intensities`` (None without orders)."""
+    with the masks of ``harmonic_intensities`` (None without orders).
+    noalloc: through ggb_spectrum_noalloc (no device allocation, legacy
+    stream, same results bit for bit; calls are serialised here)."""
     from . import _lib
     lib = _lib.load()
     x = np.ascontiguousarray(signal, dtype=np.float64)
@@ -79,12 +93,18 @@ def device_spectrum(signal, dt, differentiate=False, beta=24.0, n_pad=8, omega0=
     power = np.empty(M) if want_power else None
     ords = None if orders is None else np.ascontiguousarray(orders, dtype=np.float64)
     inten = None if ords is None else np.empty(ords.size)
-    rc = lib.ggb_spectrum(int(device), _lib.dptr(x), n, float(dt), int(bool(differentiate)),
-                          float(beta), int(n_pad), float(omega0 or 0.0),
-                          None if ords is None else _lib.dptr(ords),
-                          0 if ords is None else ords.size,
-                          None if power is None else _lib.dptr(power),
-                          None if inten is None else _lib.dptr(inten))
-    _lib.check(rc, None, "ggb_spectrum")
+    args = (int(device), _lib.dptr(x), n, float(dt), int(bool(differentiate)),
+            float(beta), int(n_pad), float(omega0 or 0.0),
+            None if ords is None else _lib.dptr(ords),
+            0 if ords is None else ords.size,
+            None if power is None else _lib.dptr(power),
+            None if inten is None else _lib.dptr(inten))
+    if noalloc:
+        with _noalloc_lock:
+            rc = lib.ggb_spectrum_noalloc(*args)
+        _lib.check(rc, None, "ggb_spectrum_noalloc")
+    else:
+        rc = lib.ggb_spectrum(*args)
+        _lib.check(rc, None, "ggb_spectrum")
     w = 2.0 * np.pi * (np.arange(M) - M // 2) / (M * dt)
     return w, power, inten
