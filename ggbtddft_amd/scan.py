@@ -57,8 +57,9 @@ class ScanGroup:
     """A batched scan served as one batch on one GPU: one lane (context) per
     static key with a slot per request of that key, the chunks of all busy
     lanes advanced by one hipGraph launch (engine.Group), requests seated in
-    free slots at chunk boundaries until all have run their own number of
-    steps.  No host threads: one loop launches every lane's chunk at once.
+    free slots at chunk boundaries (longest first) until all have run their
+    own number of steps.  No host threads: one loop launches every lane's
+    chunk at once.
     Results are each request's observables, Ehrenfest acceleration and
     spectrum (as the serving loop's)."""
 
@@ -76,7 +77,10 @@ class ScanGroup:
         request order, stats)."""
         t0 = time.perf_counter()
         waiting = {}
-        for r in requests:
+        # longest first within each lane (all requests are there at the start):
+        # the short ones fill the slots the long ones leave, so a lane's last
+        # chunks are not one long request running alone
+        for r in sorted(requests, key=lambda r: -r.pulse().n_steps(r.dt)):
             waiting.setdefault(r.key(), []).append(r)
         lanes = {k: Lane(self.systems[k], min(self.slots, len(v)), self.chunk, self.ring,
                          self.factory, self.device) for k, v in waiting.items()}
