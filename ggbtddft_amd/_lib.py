@@ -22,8 +22,8 @@ EXPORTS = ("ggb_abi_version", "ggb_create", "ggb_upload_static", "ggb_upload_orb
            "ggb_rebuild_potential", "ggb_footprint", "ggb_selftest_copy", "ggb_set_graph",
            "ggb_spectrum", "ggb_set_kinetic_variant", "ggb_autotune", "ggb_ground_state",
            "ggb_set_channels", "ggb_get_channels", "ggb_spectrum_noalloc", "ggb_group_create",
-           "ggb_group_run", "ggb_group_last_error", "ggb_group_destroy", "ggb_last_error",
-           "ggb_destroy")
+           "ggb_group_run", "ggb_group_last_error", "ggb_group_destroy", "ggb_propagator_tables",
+           "ggb_last_error", "ggb_destroy")
 KIN_VARIANTS = ("default", "narrow", "tiled16", "tiled32", "split16")
 NPHASE = 7
 PHASES = ("half_potential_laser", "kinetic_qt", "kinetic_q", "laser_half_potential_obs",
@@ -124,6 +124,9 @@ def load():
     lib.ggb_ground_state.restype = ctypes.c_int
     lib.ggb_ground_state.argtypes = [ctypes.POINTER(GsProblem), _ip, _dp, _dp, _dp, _dp, _dp, _dp,
                                      _dp, _dp, _dp, _dp, _ip, _ip]
+    lib.ggb_propagator_tables.restype = ctypes.c_int
+    lib.ggb_propagator_tables.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp,
+                                          ctypes.c_double, ctypes.c_int32, _dp, _dp, _ip, _ip]
     lib.ggb_set_channels.restype = ctypes.c_int
     lib.ggb_set_channels.argtypes = [vp, ctypes.c_int32]
     lib.ggb_get_channels.restype = ctypes.c_int

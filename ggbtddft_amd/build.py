@@ -10,7 +10,9 @@ OUT = os.path.join(HERE, "libggbtd.so")
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
                # keep the compiler from forming packed FP32 instructions
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-               "-Wno-pass-failed"]
+               "-Wno-pass-failed",
+               # the propagator tables' batched eigendecomposition (rocSOLVER on rocBLAS)
+               "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 
 def build(verbose=False):

@@ -34,6 +34,8 @@
 
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -3503,6 +3505,137 @@ extern "C" int ggb_autotune(ggb_ctx *ctx, int32_t reps, int32_t *chosen, double 
     if (rc == GGB_OK && chosen) *chosen = best;
     if (rc == GGB_OK && ms)
         for (int v = 0; v < GGB_KIN_NVARIANT; ++v) ms[v] = t[v];
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Kinetic propagator tables on the device: the radial Hamiltonians
+// H_l = -T/2 + diag(l(l+1)/2r^2 + v) of every l (column-major, one matrix per
+// l), their eigenpairs by one batched divide-and-conquer call (rocSOLVER
+// dsyevd, FP64, ascending eigenvalues), then the eigenvectors with eigenvalue
+// <= ecut packed as Q[l][i][a] (a < nk_cap, zero beyond nkeep[l]).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_build_hl(const double *__restrict__ T, const double *__restrict__ v,
+                                                 const double *__restrict__ r, double *__restrict__ H, int N) {
+    const size_t idx = (size_t)blockIdx.x * WG + threadIdx.x;  // element (i, j) at j N + i
+    const int l = blockIdx.y;
+    if (idx >= (size_t)N * N) return;
+    const int j = (int)(idx / N), i = (int)(idx % N);
+    double h = -0.5 * T[(size_t)i * N + j];
+    if (i == j) h += 0.5 * l * (l + 1.0) / (r[i] * r[i]) + v[i];
+    H[(size_t)l * N * N + idx] = h;
+}
+
+__global__ __launch_bounds__(WG) void k_pack_q(const double *__restrict__ V, const int32_t *__restrict__ nkeep,
+                                               double *__restrict__ Q, int N, int nk_cap) {
+    const size_t idx = (size_t)blockIdx.x * WG + threadIdx.x;  // Q[l][i][a], a fastest
+    const int l = blockIdx.y;
+    if (idx >= (size_t)N * nk_cap) return;
+    const int i = (int)(idx / nk_cap), a = (int)(idx % nk_cap);
+    Q[(size_t)l * N * nk_cap + idx] = a < nkeep[l] ? V[(size_t)l * N * N + (size_t)a * N + i] : 0.0;
+}
+
+extern "C" int ggb_propagator_tables(int32_t device, int32_t n_r, int32_t n_l, const double *r,
+                                     const double *T, const double *v, double ecut, int32_t nk_cap,
+                                     double *Q, double *lam, int32_t *nkeep, int32_t *nk_needed) {
+    auto bad = [](const char *m) {
+        g_create_err = m;
+        return GGB_E_ARG;
+    };
+    if (!r || !T || !v || !Q || !lam || !nkeep) return bad("ggb_propagator_tables: null argument");
+    if (n_r < 4 || n_r > 4096 || n_l < 1 || n_l > 1024 || nk_cap < 1 || nk_cap > n_r + 15)
+        return bad("ggb_propagator_tables: sizes out of range");
+    if (hipSetDevice(device) != hipSuccess) {
+        g_create_err = "ggb_propagator_tables: hipSetDevice failed";
+        return GGB_E_HIP;
+    }
+    const size_t N = (size_t)n_r, NN = N * N;
+    double *dT = nullptr, *dv = nullptr, *dr = nullptr, *dH = nullptr, *dD = nullptr, *dE = nullptr,
+           *dQ = nullptr;
+    int32_t *dk = nullptr;
+    rocblas_int *dinfo = nullptr;
+    hipStream_t st = nullptr;
+    rocblas_handle hb = nullptr;
+    int rc = GGB_OK;
+    std::string err;
+    auto hipok = [&](hipError_t e, const char *what) {
+        if (e == hipSuccess) return true;
+        if (rc == GGB_OK) {
+            rc = (e == hipErrorOutOfMemory) ? GGB_E_NOMEM : GGB_E_HIP;
+            err = std::string("ggb_propagator_tables: ") + what + ": " + hipGetErrorString(e);
+        }
+        return false;
+    };
+    std::vector<double> D((size_t)n_l * N);
+    std::vector<int32_t> keep(n_l);
+    std::vector<rocblas_int> info(n_l, 0);
+    if (hipok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream") &&
+        hipok(hipMalloc((void **)&dT, NN * 8), "alloc") && hipok(hipMalloc((void **)&dv, N * 8), "alloc") &&
+        hipok(hipMalloc((void **)&dr, N * 8), "alloc") &&
+        hipok(hipMalloc((void **)&dH, (size_t)n_l * NN * 8), "alloc") &&
+        hipok(hipMalloc((void **)&dD, (size_t)n_l * N * 8), "alloc") &&
+        hipok(hipMalloc((void **)&dE, (size_t)n_l * N * 8), "alloc") &&
+        hipok(hipMalloc((void **)&dinfo, (size_t)n_l * sizeof(rocblas_int)), "alloc") &&
+        hipok(hipMemcpyAsync(dT, T, NN * 8, hipMemcpyHostToDevice, st), "copy") &&
+        hipok(hipMemcpyAsync(dv, v, N * 8, hipMemcpyHostToDevice, st), "copy") &&
+        hipok(hipMemcpyAsync(dr, r, N * 8, hipMemcpyHostToDevice, st), "copy")) {
+        k_build_hl<<<dim3((unsigned)((NN + WG - 1) / WG), n_l), WG, 0, st>>>(dT, dv, dr, dH, n_r);
+        if (hipok(hipGetLastError(), "build")) {
+            rocblas_status bs = rocblas_create_handle(&hb);
+            if (bs == rocblas_status_success) bs = rocblas_set_stream(hb, st);
+            if (bs == rocblas_status_success)
+                bs = rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, n_r, dH,
+                                                      n_r, (rocblas_stride)NN, dD, (rocblas_stride)N, dE,
+                                                      (rocblas_stride)N, dinfo, n_l);
+            if (bs != rocblas_status_success) {
+                rc = GGB_E_HIP;
+                err = std::string("ggb_propagator_tables: rocsolver_dsyevd: ") + rocblas_status_to_string(bs);
+            }
+        }
+        if (rc == GGB_OK && hipok(hipMemcpyAsync(D.data(), dD, D.size() * 8, hipMemcpyDeviceToHost, st), "copy") &&
+            hipok(hipMemcpyAsync(info.data(), dinfo, info.size() * sizeof(rocblas_int), hipMemcpyDeviceToHost, st),
+                  "copy") &&
+            hipok(hipStreamSynchronize(st), "sync")) {
+            int need = 1;
+            for (int l = 0; l < n_l && rc == GGB_OK; ++l) {
+                if (info[l] != 0) {
+                    rc = GGB_E_HIP;
+                    err = "ggb_propagator_tables: no convergence for l = " + std::to_string(l);
+                    break;
+                }
+                int k = 0;
+                while (k < n_r && D[(size_t)l * N + k] <= ecut) ++k;  // ascending
+                keep[l] = k;
+                need = std::max(need, k);
+            }
+            const int need16 = ((need + 15) / 16) * 16;
+            if (nk_needed) *nk_needed = need16;
+            if (rc == GGB_OK && need > nk_cap) {
+                rc = GGB_E_ARG;
+                err = "ggb_propagator_tables: nk_cap below the largest kept count (" + std::to_string(need16) + ")";
+            }
+            if (rc == GGB_OK && hipok(hipMalloc((void **)&dk, (size_t)n_l * 4), "alloc") &&
+                hipok(hipMalloc((void **)&dQ, (size_t)n_l * N * nk_cap * 8), "alloc") &&
+                hipok(hipMemcpyAsync(dk, keep.data(), (size_t)n_l * 4, hipMemcpyHostToDevice, st), "copy")) {
+                k_pack_q<<<dim3((unsigned)((N * nk_cap + WG - 1) / WG), n_l), WG, 0, st>>>(dH, dk, dQ, n_r, nk_cap);
+                if (hipok(hipGetLastError(), "pack") &&
+                    hipok(hipMemcpyAsync(Q, dQ, (size_t)n_l * N * nk_cap * 8, hipMemcpyDeviceToHost, st), "copy") &&
+                    hipok(hipStreamSynchronize(st), "sync")) {
+                    for (int l = 0; l < n_l; ++l) {
+                        nkeep[l] = keep[l];
+                        for (int a = 0; a < nk_cap; ++a) lam[(size_t)l * nk_cap + a] = a < keep[l] ? D[(size_t)l * N + a] : 0.0;
+                    }
+                }
+            }
+        }
+    }
+    if (hb) (void)rocblas_destroy_handle(hb);
+    if (st) (void)hipStreamSynchronize(st);
+    for (void *q : {(void *)dT, (void *)dv, (void *)dr, (void *)dH, (void *)dD, (void *)dE, (void *)dQ, (void *)dk,
+                    (void *)dinfo})
+        if (q) (void)hipFree(q);
+    if (st) (void)hipStreamDestroy(st);
+    if (rc != GGB_OK) g_create_err = err;
     return rc;
 }
 

@@ -318,9 +318,10 @@ class Server:
         self.lanes_closed = 0
 
     def _make_system(self, key):
-        """Static data of a key: the ground state on the device (which
-        allocates, so it holds the device alone; lanes wait meanwhile), then
-        the eigenpairs on the host while the lanes run."""
+        """Static data of a key: the ground state and the propagator tables'
+        eigenpairs on the device (both allocate, so each holds the device
+        alone; lanes wait meanwhile), the angular tables on the host between
+        them while the lanes run."""
         if not self._device_gs:
             return AtomSystem(key[0], key[1], key[2], key[3], key[4])
         from .atoms import Atom
@@ -329,7 +330,10 @@ class Server:
         grid = RadialGrid(key[1], key[2])
         with self._gate.exclusive():
             gs = DeviceGroundState(grid, Atom(key[0]), device=self.device)
-        return AtomSystem(key[0], key[1], key[2], key[3], key[4], ground_state=gs)
+        sysm = AtomSystem(key[0], key[1], key[2], key[3], key[4], ground_state=gs, tables=None)
+        with self._gate.exclusive():  # the batched eigendecomposition allocates
+            sysm.build_tables("device", self.device)
+        return sysm
 
     def _system(self, key):
         if key not in self.systems:

@@ -54,7 +54,8 @@ def _round16(n):
 
 class AtomSystem:
     def __init__(self, atom, N, r_max, lmax, dt, zeta=1.0, n_theta=None, l_h=None, seed=0,
-                 cap=True, interacting=None, ecut_factor=1.0, ground_state=None, gs_device=0):
+                 cap=True, interacting=None, ecut_factor=1.0, ground_state=None, gs_device=0,
+                 tables="host"):
         self.atom = atom if isinstance(atom, Atom) else Atom(atom)
         self.N, self.r_max, self.lmax, self.dt = int(N), float(r_max), int(lmax), float(dt)
         self.n_l = self.lmax + 1
@@ -79,6 +80,27 @@ class AtomSystem:
         self.interacting = self.gs.interacting
         self.v_gs = self.gs.v_gs
         self.ecut = ecut_factor * math.pi / self.dt
+        # kinetic propagator tables: "host" (numpy eigh per l), "device"
+        # (ggb_propagator_tables: one batched eigendecomposition on GPU
+        # gs_device), or None (the caller runs build_tables later)
+        if tables is not None:
+            self.build_tables(tables, gs_device)
+        self.W = manolopoulos_cap(self.grid.r, self.r_max / 1.5, self.r_max) if cap else np.zeros(self.N)
+        self.mask = np.exp(-0.5 * self.dt * self.W)
+        self.psi0 = np.zeros((self.n_l, self.n_orb, self.N), dtype=np.complex128)
+        for k, (n, l, m, occ, s) in enumerate(orbs):
+            self.psi0[l, k] = self.gs.u[s]
+
+    def build_tables(self, where="host", device=0):
+        """Eigenpairs of every H_l with lambda <= E_cut: Q, lam, nkeep, nk_max."""
+        if where == "device":
+            self._tables_device(device)
+        elif where == "host":
+            self._tables_host()
+        else:
+            raise ValueError("tables: 'host' or 'device'")
+
+    def _tables_host(self):
         def eig_l(l):
             lam, Q = np.linalg.eigh(self.grid.hamiltonian(l, self.atom.Z, self.v_gs))
             nk = int(np.searchsorted(lam, self.ecut, side="right"))
@@ -97,11 +119,29 @@ class AtomSystem:
         for l, (lam, Q) in enumerate(eig):
             self.Q[l, :, :lam.size] = Q
             self.lam[l, :lam.size] = lam
-        self.W = manolopoulos_cap(self.grid.r, self.r_max / 1.5, self.r_max) if cap else np.zeros(self.N)
-        self.mask = np.exp(-0.5 * self.dt * self.W)
-        self.psi0 = np.zeros((self.n_l, self.n_orb, self.N), dtype=np.complex128)
-        for k, (n, l, m, occ, s) in enumerate(orbs):
-            self.psi0[l, k] = self.gs.u[s]
+
+    def _tables_device(self, device):
+        """The same eigenpairs from one batched FP64 eigendecomposition on the
+        GPU (H_l formed there from T, r and v = -Z/r + v_gs)."""
+        from . import _lib
+        lib = _lib.load()
+        g = self.grid
+        cap = _round16(self.N)
+        T = np.ascontiguousarray(g.T, dtype=np.float64)
+        v = np.ascontiguousarray(-self.atom.Z / g.r + self.v_gs, dtype=np.float64)
+        r = np.ascontiguousarray(g.r, dtype=np.float64)
+        Q = np.empty((self.n_l, self.N, cap))
+        lam = np.empty((self.n_l, cap))
+        nkeep = np.empty(self.n_l, dtype=np.int32)
+        need = np.zeros(1, dtype=np.int32)
+        rc = lib.ggb_propagator_tables(int(device), self.N, self.n_l, _lib.dptr(r), _lib.dptr(T),
+                                       _lib.dptr(v), float(self.ecut), cap, _lib.dptr(Q),
+                                       _lib.dptr(lam), _lib.iptr(nkeep), _lib.iptr(need))
+        _lib.check(rc, None, "ggb_propagator_tables")
+        self.nkeep = nkeep
+        self.nk_max = _round16(max(1, int(nkeep.max())))
+        self.Q = np.ascontiguousarray(Q[:, :, :self.nk_max])
+        self.lam = np.ascontiguousarray(lam[:, :self.nk_max])
 
     def static_arrays(self):
         """Contiguous arrays in the layouts of ggb_static (include/ggbtd.h)."""

@@ -335,6 +335,21 @@ int ggb_ground_state(const ggb_gs_problem *p, const int32_t *shell_l, const doub
                      const double *lam0, double *u, double *eps, double *v_gs, double *rho_u,
                      int32_t *iterations, int32_t *converged);
 
+/* Kinetic propagator tables on the device.  For l = 0..n_l-1 the radial
+ * Hamiltonian H_l = -T/2 + diag(l(l+1)/(2 r^2) + v) (T [n_r][n_r] symmetric,
+ * v = -Z/r + v_gs) is formed on the device and all n_l are diagonalised by
+ * one batched FP64 divide-and-conquer call (rocSOLVER dsyevd); the
+ * eigenpairs with eigenvalue <= ecut (ascending) are returned as
+ * Q[n_l][n_r][nk_cap] (column a = the a-th kept eigenvector, zero beyond
+ * nkeep[l]; any sign), lam[n_l][nk_cap] and nkeep[n_l].  *nk_needed (may be
+ * NULL) gets the largest kept count rounded up to 16; GGB_E_ARG when it
+ * exceeds nk_cap.  Errors: ggb_last_error(NULL).  Replaces the host
+ * precompute of the propagators (the reference's propagatorsR,
+ * internal.py:232-333, which writes Crank-Nicolson tables per l). */
+int ggb_propagator_tables(int32_t device, int32_t n_r, int32_t n_l, const double *r,
+                          const double *T, const double *v, double ecut, int32_t nk_cap,
+                          double *Q, double *lam, int32_t *nkeep, int32_t *nk_needed);
+
 /* Message of the last failure on ctx (or of the last failed ggb_create when
  * ctx is NULL). */
 const char *ggb_last_error(const ggb_ctx *ctx);

@@ -137,6 +137,17 @@ int main(void) {
     EXPECT(ggb_ground_state(&gp, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
                             &it, &conv) == GGB_E_ARG);
 
+    /* propagator tables: rejected sizes, then a valid call (no device: GGB_E_HIP) */
+    double rr[8], TT[64], vv[8], QQ[64], ll[64];
+    int32_t kk[8], need = 0;
+    for (int i = 0; i < 8; ++i) { rr[i] = 1.0 + i; vv[i] = -1.0 / rr[i]; }
+    for (int i = 0; i < 64; ++i) TT[i] = (i % 9 == 0) ? -2.0 : 0.0;
+    EXPECT(ggb_propagator_tables(0, 8, 1, NULL, TT, vv, 10.0, 8, QQ, ll, kk, &need) == GGB_E_ARG);
+    EXPECT(ggb_propagator_tables(0, 2, 1, rr, TT, vv, 10.0, 8, QQ, ll, kk, &need) == GGB_E_ARG);
+    EXPECT(ggb_propagator_tables(0, 8, 1, rr, TT, vv, 10.0, 0, QQ, ll, kk, &need) == GGB_E_ARG);
+    rc = ggb_propagator_tables(0, 8, 1, rr, TT, vv, 10.0, 8, QQ, ll, kk, &need);
+    EXPECT(rc == GGB_OK || rc == GGB_E_HIP);
+
     /* self tests reach the device */
     double out[512];
     rc = ggb_selftest_mfma(0, out);
@@ -160,7 +171,8 @@ def test_c_abi_host_code_under_asan_ubsan(tmp_path):
         san += ["-Xarch_host", flag]  # host code only
     cmd = [HIPCC, "--offload-arch=gfx950", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", *san,
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-I", os.path.join(ROOT, "include"), SRC, "-o", str(lib)]
+           "-I", os.path.join(ROOT, "include"), SRC, "-L", os.path.join(ROCM, "lib"), "-lrocsolver",
+           "-lrocblas", "-o", str(lib)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     drv = tmp_path / "drive.c"
