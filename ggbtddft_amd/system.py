@@ -55,7 +55,7 @@ def _round16(n):
 class AtomSystem:
     def __init__(self, atom, N, r_max, lmax, dt, zeta=1.0, n_theta=None, l_h=None, seed=0,
                  cap=True, interacting=None, ecut_factor=1.0, ground_state=None, gs_device=0,
-                 tables="host"):
+                 tables="host", propagator="exp"):
         self.atom = atom if isinstance(atom, Atom) else Atom(atom)
         self.N, self.r_max, self.lmax, self.dt = int(N), float(r_max), int(lmax), float(dt)
         self.n_l = self.lmax + 1
@@ -80,6 +80,14 @@ class AtomSystem:
         self.interacting = self.gs.interacting
         self.v_gs = self.gs.v_gs
         self.ecut = ecut_factor * math.pi / self.dt
+        # kinetic propagator: "exp" = exp(-i H_l dt); "cn" = the reference's
+        # Crank-Nicolson W_l = (I + i dt/2 H_l)^{-1} (I - i dt/2 H_l)
+        # (internal.py:252-257), diagonal in the same eigenvectors with factor
+        # (1 - i lam dt/2)/(1 + i lam dt/2) = e^{-i lam' dt}, lam' = 2 atan(lam dt/2)/dt:
+        # the device applies e^{-i lam' dt}, so only the eigenvalues handed over change
+        if propagator not in ("exp", "cn"):
+            raise ValueError("propagator: 'exp' or 'cn'")
+        self.propagator = propagator
         # kinetic propagator tables: "host" (numpy eigh per l), "device"
         # (ggb_propagator_tables: one batched eigendecomposition on GPU
         # gs_device), or None (the caller runs build_tables later)
@@ -143,12 +151,21 @@ class AtomSystem:
         self.Q = np.ascontiguousarray(Q[:, :, :self.nk_max])
         self.lam = np.ascontiguousarray(lam[:, :self.nk_max])
 
+    def phase_eigenvalues(self, lam):
+        """The eigenvalues whose e^{-i lam dt} the device applies: lam itself, or
+        for the Crank-Nicolson propagator 2 atan(lam dt/2)/dt."""
+        lam = np.asarray(lam, dtype=np.float64)
+        if self.propagator == "cn":
+            return 2.0 * np.arctan(0.5 * self.dt * lam) / self.dt
+        return lam
+
     def static_arrays(self):
         """Contiguous arrays in the layouts of ggb_static (include/ggbtd.h)."""
         a = self.angular
+        lam = self.phase_eigenvalues(self.lam)
         return dict(r=np.ascontiguousarray(self.grid.r), wr=np.ascontiguousarray(self.grid.wr),
                     v_gs=np.ascontiguousarray(self.v_gs), mask=np.ascontiguousarray(self.mask),
-                    Q=np.ascontiguousarray(self.Q), lam=np.ascontiguousarray(self.lam),
+                    Q=np.ascontiguousarray(self.Q), lam=np.ascontiguousarray(lam),
                     nkeep=np.ascontiguousarray(self.nkeep), B=np.ascontiguousarray(a.B),
                     PL=np.ascontiguousarray(a.PL), chi=np.ascontiguousarray(a.x),
                     omega=np.ascontiguousarray(a.omega), m=np.ascontiguousarray(self.m),
@@ -159,7 +176,8 @@ class AtomSystem:
         state, up to a global phase): real radial function phi [n_orb, N] in u
         units with unit norm, its l and its
         energy eps = sum_a lam_a |<q_a|phi>|^2 over the retained eigenpairs
-        (phi is an eigenvector of its block, so this is its eigenvalue)."""
+        (phi is an eigenvector of its block, so this is its eigenvalue; with
+        the Crank-Nicolson propagator the eigenvalue the device's phase uses)."""
         orbs = self.atom.orbitals
         phi = np.zeros((self.n_orb, self.N))
         l0 = np.zeros(self.n_orb, dtype=np.int32)
@@ -174,7 +192,7 @@ class AtomSystem:
             l0[k] = l
             c = self.Q[l, :, :self.nkeep[l]].T @ u
             eps[k] = float((self.lam[l, :self.nkeep[l]] * c * c).sum() / (c * c).sum())
-        return dict(phi=phi, l0=l0, eps=eps)
+        return dict(phi=phi, l0=l0, eps=self.phase_eigenvalues(eps))
 
     def total_acceleration(self, acc, norm, E):
         """Ehrenfest dipole acceleration of all electrons,

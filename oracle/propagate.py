@@ -60,7 +60,7 @@ class System:
     """Static data of one atom on one grid (everything a run needs but the field)."""
 
     def __init__(self, atom, N, r_max, lmax, dt, zeta=1.0, n_theta=None, l_h=None,
-                 seed=0, cap=True, interacting=None, gs=None, ecut_factor=1.0):
+                 seed=0, cap=True, interacting=None, gs=None, ecut_factor=1.0, propagator="exp"):
         self.atom = atoms.canonical(atom)
         self.Z = atoms.nuclear_charge(self.atom)
         self.N, self.r_max, self.lmax, self.dt, self.zeta = N, r_max, lmax, dt, zeta
@@ -86,6 +86,9 @@ class System:
         self.gs = gs
         self.v_gs = gs["v_gs"]
         self.ecut = ecut_factor * np.pi / dt
+        if propagator not in ("exp", "cn"):
+            raise ValueError("propagator: 'exp' or 'cn'")
+        self.propagator = propagator
         self.Q, self.lam, self.nkeep = [], [], []
         for l in range(self.nl):
             H = meshmod.radial_hamiltonian(m, l, self.Z, self.v_gs)
@@ -169,9 +172,19 @@ class System:
                 psi[l, k] = cs * x0 - 1j * sn * x1
                 psi[l + 1, k] = -1j * sn * x0 + cs * x1
 
+    def kinetic_phase(self, lam):
+        """Eigenvalue factor of the kinetic propagator: e^{-i lam dt}, or for the
+        Crank-Nicolson propagator of the reference (propagatorsR,
+        internal.py:252-257: W = (I + i dt/2 H)^{-1} (I - i dt/2 H), the same
+        eigenvectors) (1 - i lam dt/2) / (1 + i lam dt/2)."""
+        if self.propagator == "cn":
+            x = 0.5 * self.dt * lam
+            return (1.0 - 1j * x) / (1.0 + 1j * x)
+        return np.exp(-1j * self.dt * lam)
+
     def kinetic(self, psi):
         for l in range(self.nl):
-            ph = np.exp(-1j * self.dt * self.lam[l])
+            ph = self.kinetic_phase(self.lam[l])
             Y = self.Q[l].T @ psi[l].T               # [nkeep, k]
             psi[l] = (self.Q[l] @ (ph[:, None] * Y)).T
 

@@ -238,3 +238,58 @@ def test_channel_dipoles_sum_to_theta_space_dipole():
     # l + 1, so the imaginary parts cancel in the sum
     assert np.abs(chan.imag.sum(axis=1)).max() < 1e-13 * np.abs(dip).max()
     assert np.all(chan[S.m > 0, 0] == 0.0)
+
+
+# ---------------------------------------------------------------------------
+# Crank-Nicolson kinetic propagator (the reference's propagatorsR,
+# internal.py:252-257, restated): W_l = (I + i dt/2 H_l)^{-1} (I - i dt/2 H_l)
+# by a dense solve, against the oracle's eigenvector form without the cut.
+# ---------------------------------------------------------------------------
+def _cn_system(propagator):
+    return P.System("He", 32, 20.0, 2, 0.05, ecut_factor=1e9, propagator=propagator)
+
+
+def _dense_cn(S, l):
+    from oracle import mesh as M
+    H = M.radial_hamiltonian(S.mesh, l, S.Z, S.v_gs)
+    eye = np.eye(S.N)
+    return np.linalg.solve(eye + 0.5j * S.dt * H, eye - 0.5j * S.dt * H)
+
+
+def test_crank_nicolson_kinetic_is_the_dense_solve():
+    S = _cn_system("cn")
+    assert all(nk == S.N for nk in S.nkeep)  # no cut: the whole propagator
+    rng = np.random.default_rng(2)
+    psi = rng.normal(size=(S.nl, S.norb, S.N)) + 1j * rng.normal(size=(S.nl, S.norb, S.N))
+    out = psi.copy()
+    S.kinetic(out)
+    for l in range(S.nl):
+        want = (_dense_cn(S, l) @ psi[l].T).T
+        assert np.abs(out[l] - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_crank_nicolson_check_separates_it_from_the_exponential():
+    # the exponential form differs from W_l by O((lam dt)^3): the check above
+    # would fail it
+    S = _cn_system("exp")
+    rng = np.random.default_rng(2)
+    psi = rng.normal(size=(S.nl, S.norb, S.N)) + 1j * rng.normal(size=(S.nl, S.norb, S.N))
+    out = psi.copy()
+    S.kinetic(out)
+    want = (_dense_cn(S, 0) @ psi[0].T).T
+    assert np.abs(out[0] - want).max() > 1e-3 * np.abs(want).max()
+
+
+def test_product_crank_nicolson_phases():
+    """The product hands the device eigenvalues lam' = 2 atan(lam dt/2)/dt,
+    whose e^{-i lam' dt} is the Crank-Nicolson factor (1 - i lam dt/2)/(1 + i lam dt/2)."""
+    from ggbtddft_amd.system import AtomSystem
+    S = AtomSystem("He", 32, 20.0, 2, 0.05, propagator="cn")
+    lam_dev = S.static_arrays()["lam"]
+    for l in range(S.n_l):
+        k = int(S.nkeep[l])
+        x = 0.5 * S.dt * S.lam[l, :k]
+        cn = (1.0 - 1j * x) / (1.0 + 1j * x)
+        assert np.abs(np.exp(-1j * S.dt * lam_dev[l, :k]) - cn).max() < 1e-14
+    with pytest.raises(ValueError):
+        AtomSystem("He", 32, 20.0, 2, 0.05, propagator="cayley")
