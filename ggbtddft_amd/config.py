@@ -104,6 +104,12 @@ class RunConfig:
         r_mask = max(float(k["ABC_R_SCALE"]) * self.E0 / self.omega0 ** 2, 16.0)
         self.r_mask = float(k.get("R_MASK", r_mask))
         self.r_max = 1.5 * self.r_mask
+        # EXP_METHOD (init.py:55-58, 189; internal.py:252-318): CN = 0, CN2 = 1,
+        # CN3 = 2, EIG = 3 (the default), anything else expm; EIG and expm are
+        # both exp(-i H dt)
+        em = k["EXP_METHOD"]
+        em = {0: "CN", 1: "CN2", 2: "CN3", 3: "EIG"}.get(em, em) if isinstance(em, (int, float)) else str(em)
+        self.propagator = {"CN": "cn", "CN2": "cn2", "CN3": "cn3"}.get(str(em).upper(), "exp")
 
     @classmethod
     def from_file(cls, path):

@@ -37,7 +37,7 @@ def run(cfg, out_root, working_root=None, index=0, precision=0, device=0, factor
     gpu = factory is None
     system = AtomSystem(cfg.atom, cfg.N, cfg.r_max, cfg.lmax, cfg.dt, zeta=cfg.zeta,
                         interacting=cfg.interacting, ground_state="device" if gpu else None,
-                        gs_device=device)
+                        gs_device=device, propagator=cfg.propagator)
     if gpu:
         from .engine import Propagator
         factory = functools.partial(Propagator, precision=precision)

@@ -25,6 +25,9 @@ from .grid import RadialGrid
 from .groundstate import DeviceGroundState, GroundState
 
 
+PROPAGATORS = ("exp", "cn", "cn2", "cn3")
+
+
 def manolopoulos_cap(r, r_mask, r_max):
     delta, c = 0.20, 2.62206
     a = 1.0 - 16.0 / c ** 3
@@ -80,13 +83,15 @@ class AtomSystem:
         self.interacting = self.gs.interacting
         self.v_gs = self.gs.v_gs
         self.ecut = ecut_factor * math.pi / self.dt
-        # kinetic propagator: "exp" = exp(-i H_l dt); "cn" = the reference's
-        # Crank-Nicolson W_l = (I + i dt/2 H_l)^{-1} (I - i dt/2 H_l)
-        # (internal.py:252-257), diagonal in the same eigenvectors with factor
-        # (1 - i lam dt/2)/(1 + i lam dt/2) = e^{-i lam' dt}, lam' = 2 atan(lam dt/2)/dt:
+        # kinetic propagator: "exp" = exp(-i H_l dt) (the reference's EIG / expm);
+        # "cn", "cn2", "cn3" = its Crank-Nicolson forms W_l = Wp^{-1} Wm with Wp,
+        # Wm the truncated series of exp(+-i dt H_l / 2) to order 1, 2, 3
+        # (propagatorsR, internal.py:252-318).  All are diagonal in the
+        # eigenvectors of H_l with factor conj(p(x))/p(x) = e^{-i lam' dt},
+        # p(x) = sum_{k<=n} (i x)^k / k!, x = lam dt/2, lam' = 2 arg(p(x)) / dt:
         # the device applies e^{-i lam' dt}, so only the eigenvalues handed over change
-        if propagator not in ("exp", "cn"):
-            raise ValueError("propagator: 'exp' or 'cn'")
+        if propagator not in PROPAGATORS:
+            raise ValueError("propagator: one of %s" % (PROPAGATORS,))
         self.propagator = propagator
         # kinetic propagator tables: "host" (numpy eigh per l), "device"
         # (ggb_propagator_tables: one batched eigendecomposition on GPU
@@ -153,11 +158,21 @@ class AtomSystem:
 
     def phase_eigenvalues(self, lam):
         """The eigenvalues whose e^{-i lam dt} the device applies: lam itself, or
-        for the Crank-Nicolson propagator 2 atan(lam dt/2)/dt."""
+        for the Crank-Nicolson forms 2 arg(p(lam dt/2)) / dt."""
         lam = np.asarray(lam, dtype=np.float64)
-        if self.propagator == "cn":
-            return 2.0 * np.arctan(0.5 * self.dt * lam) / self.dt
-        return lam
+        if self.propagator == "exp":
+            return lam
+        n = {"cn": 1, "cn2": 2, "cn3": 3}[self.propagator]
+        x = 0.5 * self.dt * lam
+        re = np.ones_like(x)
+        im = np.zeros_like(x)
+        if n >= 1:
+            im = im + x
+        if n >= 2:
+            re = re - 0.5 * x * x
+        if n >= 3:
+            im = im - x * x * x / 6.0
+        return 2.0 * np.arctan2(im, re) / self.dt
 
     def static_arrays(self):
         """Contiguous arrays in the layouts of ggb_static (include/ggbtd.h)."""

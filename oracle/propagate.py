@@ -56,6 +56,25 @@ def manolopoulos_cap(r, r_mask, r_max):
     return W
 
 
+PROPAGATORS = ("exp", "cn", "cn2", "cn3")
+
+
+def kinetic_phase(propagator, lam, dt):
+    """Factor of the kinetic propagator on an eigenpair of H_l (see
+    System.kinetic_phase)."""
+    lam = np.asarray(lam, dtype=np.float64)
+    if propagator == "exp":
+        return np.exp(-1j * dt * lam)
+    n = {"cn": 1, "cn2": 2, "cn3": 3}[propagator]
+    x = 0.5 * dt * lam
+    p = np.zeros_like(x, dtype=np.complex128)
+    term = np.ones_like(x, dtype=np.complex128)
+    for k in range(n + 1):
+        p = p + term
+        term = term * (1j * x) / (k + 1)
+    return np.conj(p) / p
+
+
 class System:
     """Static data of one atom on one grid (everything a run needs but the field)."""
 
@@ -86,8 +105,8 @@ class System:
         self.gs = gs
         self.v_gs = gs["v_gs"]
         self.ecut = ecut_factor * np.pi / dt
-        if propagator not in ("exp", "cn"):
-            raise ValueError("propagator: 'exp' or 'cn'")
+        if propagator not in PROPAGATORS:
+            raise ValueError("propagator: one of %s" % (PROPAGATORS,))
         self.propagator = propagator
         self.Q, self.lam, self.nkeep = [], [], []
         for l in range(self.nl):
@@ -173,14 +192,12 @@ class System:
                 psi[l + 1, k] = -1j * sn * x0 + cs * x1
 
     def kinetic_phase(self, lam):
-        """Eigenvalue factor of the kinetic propagator: e^{-i lam dt}, or for the
-        Crank-Nicolson propagator of the reference (propagatorsR,
-        internal.py:252-257: W = (I + i dt/2 H)^{-1} (I - i dt/2 H), the same
-        eigenvectors) (1 - i lam dt/2) / (1 + i lam dt/2)."""
-        if self.propagator == "cn":
-            x = 0.5 * self.dt * lam
-            return (1.0 - 1j * x) / (1.0 + 1j * x)
-        return np.exp(-1j * self.dt * lam)
+        """Eigenvalue factor of the kinetic propagator: e^{-i lam dt} ("exp",
+        the reference's EIG/expm), or for its Crank-Nicolson forms CN, CN2,
+        CN3 (propagatorsR, internal.py:252-318: W = Wp^{-1} Wm with Wp, Wm the
+        truncated series of e^{+-i dt H / 2}, the same eigenvectors)
+        conj(p(x)) / p(x), p(x) = sum_{k<=n} (i x)^k / k!, x = lam dt / 2."""
+        return kinetic_phase(self.propagator, lam, self.dt)
 
     def kinetic(self, psi):
         for l in range(self.nl):

@@ -119,15 +119,15 @@ def test_short_run(atom, N, R, lmax, nsteps):
     assert np.abs(psi - ref["psi"]).max() < 1e-10
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_crank_nicolson_propagator(precision):
-    """The reference's Crank-Nicolson kinetic propagator (propagator="cn"):
-    the device applies e^{-i lam' dt} with lam' = 2 atan(lam dt/2)/dt; against
-    the oracle's Crank-Nicolson run (He on the helium configuration's grid,
-    60 steps): FP64 to 1e-10 / norms 1e-12, the FP32 mixed form to 1e-4 / 1e-7."""
-    _say("case: Crank-Nicolson propagator, He N=96 Lmax=12, precision %d" % precision)
+@pytest.mark.parametrize("propagator,precision", [("cn", 0), ("cn", 1), ("cn2", 0), ("cn3", 0)])
+def test_crank_nicolson_propagator(propagator, precision):
+    """The reference's Crank-Nicolson kinetic propagators (CN, CN2, CN3): the
+    device applies e^{-i lam' dt} with lam' = 2 arg(p(lam dt/2))/dt; against the
+    oracle's run of the same form (He on the helium configuration's grid, 60
+    steps): FP64 to 1e-10 / norms 1e-12, the FP32 mixed form to 1e-4 / 1e-7."""
+    _say("case: %s propagator, He N=96 Lmax=12, precision %d" % (propagator, precision))
     dt, nsteps = 0.05, 60
-    S, O = _pair("He", 96, 60.0, 12, dt, propagator="cn")
+    S, O = _pair("He", 96, 60.0, 12, dt, propagator=propagator)
     w = OF.omega_from_wavelength(800.0)
     Em = OF.midpoint_series(OF.e0_from_intensity(2e14), w, 2, dt)[:nsteps] + 0.02
     ref = O.run(Em)

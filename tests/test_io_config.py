@@ -36,6 +36,17 @@ def test_init_reader_matches_reference_conventions():
     assert abs(c.r_max - 1.5 * c.E0 / 0.0253 ** 2) < 1e-9
     t = c.time_grid()
     assert t[0] == -0.5 * c.n_t * c.dt and abs(t[1] - t[0] - c.dt) < 1e-12
+    assert c.propagator == "cn"                            # EXP_METHOD = CN
+
+
+@pytest.mark.parametrize("value,prop", [("CN", "cn"), ("CN2", "cn2"), ("CN3", "cn3"), ("EIG", "exp"),
+                                        ("0", "cn"), ("1", "cn2"), ("2", "cn3"), ("3", "exp"),
+                                        ("EXPM", "exp")])
+def test_exp_method_maps_to_the_propagator(value, prop):
+    # init.py:55-58: CN = 0, CN2 = 1, CN3 = 2, EIG = 3 (default); internal.py:252-318
+    c = RunConfig(read_init("atom = He\nEXP_METHOD = %s\n" % value))
+    assert c.propagator == prop
+    assert RunConfig(read_init("atom = He\n")).propagator == "exp"
 
 
 def test_init_reader_refuses_code():

@@ -249,22 +249,37 @@ def _cn_system(propagator):
     return P.System("He", 32, 20.0, 2, 0.05, ecut_factor=1e9, propagator=propagator)
 
 
-def _dense_cn(S, l):
+def _dense_cn(S, l, order=1):
+    """W = Wp^{-1} Wm with the reference's matrices (internal.py:265-300):
+    order 1 (CN): I -+ i dt/2 H; order 2 (CN2) adds - dt^2/8 H^2; order 3
+    (CN3) adds -+ i dt^3/48 H^3 (Wm takes +i, Wp -i)."""
     from oracle import mesh as M
     H = M.radial_hamiltonian(S.mesh, l, S.Z, S.v_gs)
     eye = np.eye(S.N)
-    return np.linalg.solve(eye + 0.5j * S.dt * H, eye - 0.5j * S.dt * H)
+    dt = S.dt
+    Wm = eye - 0.5j * dt * H
+    Wp = eye + 0.5j * dt * H
+    if order >= 2:
+        H2 = H @ H
+        Wm = Wm - dt ** 2 / 8.0 * H2
+        Wp = Wp - dt ** 2 / 8.0 * H2
+    if order >= 3:
+        H3 = H @ H2
+        Wm = Wm + 1j / 48.0 * dt ** 3 * H3
+        Wp = Wp - 1j / 48.0 * dt ** 3 * H3
+    return np.linalg.solve(Wp, Wm)
 
 
-def test_crank_nicolson_kinetic_is_the_dense_solve():
-    S = _cn_system("cn")
+@pytest.mark.parametrize("propagator,order", [("cn", 1), ("cn2", 2), ("cn3", 3)])
+def test_crank_nicolson_kinetic_is_the_dense_solve(propagator, order):
+    S = _cn_system(propagator)
     assert all(nk == S.N for nk in S.nkeep)  # no cut: the whole propagator
     rng = np.random.default_rng(2)
     psi = rng.normal(size=(S.nl, S.norb, S.N)) + 1j * rng.normal(size=(S.nl, S.norb, S.N))
     out = psi.copy()
     S.kinetic(out)
     for l in range(S.nl):
-        want = (_dense_cn(S, l) @ psi[l].T).T
+        want = (_dense_cn(S, l, order) @ psi[l].T).T
         assert np.abs(out[l] - want).max() < 1e-12 * np.abs(want).max()
 
 
@@ -280,16 +295,16 @@ def test_crank_nicolson_check_separates_it_from_the_exponential():
     assert np.abs(out[0] - want).max() > 1e-3 * np.abs(want).max()
 
 
-def test_product_crank_nicolson_phases():
-    """The product hands the device eigenvalues lam' = 2 atan(lam dt/2)/dt,
-    whose e^{-i lam' dt} is the Crank-Nicolson factor (1 - i lam dt/2)/(1 + i lam dt/2)."""
+@pytest.mark.parametrize("propagator", ["exp", "cn", "cn2", "cn3"])
+def test_product_kinetic_phases(propagator):
+    """The product hands the device eigenvalues lam' whose e^{-i lam' dt} is
+    the oracle's factor of the chosen propagator."""
     from ggbtddft_amd.system import AtomSystem
-    S = AtomSystem("He", 32, 20.0, 2, 0.05, propagator="cn")
+    S = AtomSystem("He", 32, 20.0, 2, 0.05, propagator=propagator)
     lam_dev = S.static_arrays()["lam"]
     for l in range(S.n_l):
         k = int(S.nkeep[l])
-        x = 0.5 * S.dt * S.lam[l, :k]
-        cn = (1.0 - 1j * x) / (1.0 + 1j * x)
-        assert np.abs(np.exp(-1j * S.dt * lam_dev[l, :k]) - cn).max() < 1e-14
+        want = P.kinetic_phase(propagator, S.lam[l, :k], S.dt)
+        assert np.abs(np.exp(-1j * S.dt * lam_dev[l, :k]) - want).max() < 1e-13
     with pytest.raises(ValueError):
         AtomSystem("He", 32, 20.0, 2, 0.05, propagator="cayley")
