@@ -37,7 +37,7 @@ to both v0 and the propagator, an effective -2iW).
 """
 import numpy as np
 
-from . import angular, atoms, field, groundstate, hartree, mesh as meshmod, xc
+from . import angular, atoms, groundstate, hartree, mesh as meshmod, xc
 
 
 def manolopoulos_cap(r, r_mask, r_max):

@@ -1,7 +1,6 @@
 """Serving on the GPU: requests seated into running lanes (continuous
 batching, two lanes overlapping on the device) give the same results as each
 request propagated alone, within 1e-12, in FP64 and in the FP32 mixed form."""
-import sys
 
 import numpy as np
 import pytest

@@ -2,7 +2,6 @@
 pipeline of oracle/propagate.py) against closed forms and the reference's
 direct-sum formula."""
 import numpy as np
-import pytest
 from scipy.special import exp1
 
 from oracle import angular as A, hartree as H, mesh as M
