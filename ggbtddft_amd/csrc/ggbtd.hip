@@ -1534,13 +1534,122 @@ __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
     if (mo < g.M && nok) epi(z, mo, n, v);
 }
 
+// The same products with one operand panel reused across tiles: a work-group
+// takes one 16-wide tile of the narrow dimension and up to TGW_T tiles of the
+// wide one (WIDE_M: along M, A differs per tile and B is shared; else along
+// N).  The K range is split over the four waves exactly as in k_tile_gemm and
+// each wave keeps one accumulator tile per wide tile, so every element is
+// formed by the same products in the same order and the partial tiles are
+// added in the same order: the results equal k_tile_gemm's bit for bit.  The
+// shared operand is loaded once per K step instead of once per tile, and the
+// epilogue's other inputs (Epi::pre) are loaded before the products.  Measured
+// at 64 argon instances (129 multipoles and theta nodes): 3 tiles per
+// work-group beat 2, 4, 5 and 9 (fewer work-groups, more registers, lower
+// occupancy), and 3 K steps per round of loads beat 2 and 5.
+// grid (narrow tiles, groups of TGW_T wide tiles, z)
+#ifndef TGW_T
+#define TGW_T 3   // wide tiles per work-group
+#endif
+#ifndef TGW_U
+#define TGW_U 3   // K steps per round of loads
+#endif
+#ifndef TGW_G
+#define TGW_G 3   // tiles per reduction round through LDS
+#endif
+#ifndef TGW_MIN_INST
+#define TGW_MIN_INST 8  // instances from which the potential products use the panel form
+#endif
+template <class Epi, bool WIDE_M>
+__global__ __launch_bounds__(WG) void k_tile_gemm_wide(GemmOperands g, Epi epi) {
+    __shared__ double red[4][TGW_G][64][4];
+    const int z = blockIdx.z;
+    if (g.active && !g.active[z]) return;  // block-uniform
+    const int s0 = blockIdx.x * 16;            // narrow tile origin
+    const int w0 = blockIdx.y * TGW_T * 16;    // first wide tile origin
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int Wn = WIDE_M ? g.M : g.N;         // extent of the wide dimension
+    const int nt = min(TGW_T, (Wn - w0 + 15) / 16);
+    // the narrow operand (B if WIDE_M, else A): one row/column per lane
+    const int sidx = s0 + lc;
+    const bool sok = sidx < (WIDE_M ? g.N : g.M);
+    const double *Sp = WIDE_M ? g.B + z * g.sbz + (long)(sok ? sidx : 0) * g.sbn
+                              : g.A + z * g.saz + (long)(sok ? sidx : 0) * g.sam;
+    const long ssk = WIDE_M ? g.sbk : g.sak;
+    // the wide operand: lane row/column lc of tile t at w0 + 16 t + lc (offsets
+    // within one z fit in 32 bits)
+    const double *Wp = (WIDE_M ? g.A + z * g.saz : g.B + z * g.sbz) + (long)w0 * (WIDE_M ? g.sam : g.sbn);
+    const int wsi = (int)(WIDE_M ? g.sam : g.sbn), wsk = (int)(WIDE_M ? g.sak : g.sbk);
+    const int kq = (((g.K + 3) / 4 + 3) / 4) * 4;  // quarter of K, multiple of 4 (as k_tile_gemm)
+    const int kb = wave * kq, ke = min(g.K, kb + kq);
+    // wave w finishes accumulator row r = w of every tile: element (m, n) of
+    // tile t, whose other epilogue inputs are loaded now, under the products
+    const int r = wave;
+    const int ro = mfma_t<double>::row(kr, r);
+    typename Epi::Pre pre[TGW_T];
+#pragma unroll
+    for (int t = 0; t < TGW_T; ++t) {
+        const int m = WIDE_M ? w0 + 16 * t + ro : s0 + ro;
+        const int n = WIDE_M ? s0 + lc : w0 + 16 * t + lc;
+        if (t < nt && m < g.M && n < g.N) pre[t] = epi.pre(z, m, n);
+    }
+    d4_t acc[TGW_T];
+#pragma unroll
+    for (int t = 0; t < TGW_T; ++t) acc[t] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = kb; k0 < ke; k0 += 4 * TGW_U) {
+        double sv[TGW_U], wv[TGW_T][TGW_U];
+#pragma unroll
+        for (int u = 0; u < TGW_U; ++u) {
+            const int k = k0 + 4 * u + kr;
+            const bool ok = k < ke;
+            sv[u] = (ok && sok) ? Sp[(long)k * ssk] : 0.0;
+#pragma unroll
+            for (int t = 0; t < TGW_T; ++t) {
+                const int wl = 16 * t + lc;  // index within the group
+                wv[t][u] = (ok && t < nt && w0 + wl < Wn) ? Wp[wl * wsi + k * wsk] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TGW_U; ++u)
+#pragma unroll
+            for (int t = 0; t < TGW_T; ++t)
+                acc[t] = WIDE_M ? mfma64(wv[t][u], sv[u], acc[t]) : mfma64(sv[u], wv[t][u], acc[t]);
+    }
+    // partial tiles through LDS, TGW_G tiles per round (wave 0's in the last
+    // slot); wave w then finishes accumulator row r = w of each tile
+#pragma unroll
+    for (int t0 = 0; t0 < TGW_T; t0 += TGW_G) {
+        if (t0 >= nt) break;  // block-uniform
+#pragma unroll
+        for (int q = 0; q < TGW_G; ++q)
+            if (t0 + q < TGW_T)
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) red[wave > 0 ? wave - 1 : 3][q][lane][rr] = acc[t0 + q][rr];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TGW_G; ++q) {
+            const int t = t0 + q;
+            if (t < TGW_T && t < nt) {
+                const double v = ((red[3][q][lane][r] + red[0][q][lane][r]) + red[1][q][lane][r]) + red[2][q][lane][r];
+                const int m = WIDE_M ? w0 + 16 * t + ro : s0 + ro;
+                const int n = WIDE_M ? s0 + lc : w0 + 16 * t + lc;
+                if (m < g.M && n < g.N) epi.apply(z, m, n, v, pre[t]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // multipole moments q_L(i) = sum_j P_L(chi_j) rho_u(i, j): M = L, N = i, K = j
 struct QmomEpi {
     double *q;
     int N, n_har;
-    __device__ void operator()(int z, int L, int i, double v) const {
+    struct Pre {};
+    __device__ Pre pre(int, int, int) const { return Pre(); }
+    __device__ void apply(int z, int L, int i, double v, const Pre &) const {
         q[((size_t)z * n_har + L) * N + i] = v;
     }
+    __device__ void operator()(int z, int L, int i, double v) const { apply(z, L, i, v, Pre()); }
 };
 
 // ---------------------------------------------------------------------------
@@ -1681,13 +1790,26 @@ struct DvEpi {
     void *phase;
     int N, Nth, interacting, mixed;
     double dt;
-    __device__ void operator()(int z, int i, int j, double v) const {
+    // the element's other inputs, loadable before its product is done
+    struct Pre {
+        double x, g, mk;
+    };
+    __device__ Pre pre(int z, int i, int j) const {
         const size_t o = ((size_t)z * N + i) * Nth + j;
-        const double sv = interacting ? v + vxc[o] - v_gs[i] : 0.0;
+        Pre r;
+        r.x = interacting ? vxc[o] : 0.0;
+        r.g = v_gs[i];
+        r.mk = mask[i];
+        return r;
+    }
+    __device__ void operator()(int z, int i, int j, double v) const { apply(z, i, j, v, pre(z, i, j)); }
+    __device__ void apply(int z, int i, int j, double v, const Pre &pr) const {
+        const size_t o = ((size_t)z * N + i) * Nth + j;
+        const double sv = interacting ? v + pr.x - pr.g : 0.0;
         dv[o] = sv;
         double sn, cs;
         sincos(0.5 * dt * sv, &sn, &cs);
-        const double fr = mask[i] * cs, fi = -mask[i] * sn;
+        const double fr = pr.mk * cs, fi = -pr.mk * sn;
         if (mixed)
             reinterpret_cast<float2 *>(phase)[o] = make_float2((float)(fr - 1.0), (float)fi);
         else
@@ -2448,7 +2570,15 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr, MarkK mk 
         gq.N = N;
         gq.K = Nth;
         QmomEpi eq{ctx->q, N, nh};
-        k_tile_gemm<<<dim3((N + 15) / 16, (nh + 15) / 16, p.n_inst), WG, 0, st>>>(gq, eq);
+        if (p.n_inst >= TGW_MIN_INST) {
+            gq.A = ctx->PL;  // the same table as [j][L]: the 16 lanes of a K row read 16 consecutive L
+            gq.sam = 1;
+            gq.sak = nh;
+            const int ntl = (nh + 15) / 16;
+            k_tile_gemm_wide<QmomEpi, true><<<dim3((N + 15) / 16, (ntl + TGW_T - 1) / TGW_T, p.n_inst), WG, 0, st>>>(gq, eq);
+        } else {
+            k_tile_gemm<<<dim3((N + 15) / 16, (nh + 15) / 16, p.n_inst), WG, 0, st>>>(gq, eq);
+        }
         HIPCHK(ctx, mk(0));
         const int cs = (N + WG - 1) / WG;  // N <= 4096 (ggb_create): cs <= 16
         auto har = cs <= 2 ? k_hartree<2> : cs <= 4 ? k_hartree<4> : cs <= 8 ? k_hartree<8> : k_hartree<16>;
@@ -2472,7 +2602,12 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr, MarkK mk 
     gv.N = Nth;
     gv.K = p.interacting ? nh : 0;
     DvEpi ev{ctx->vxc, ctx->v_gs, ctx->mask, ctx->dv, ctx->phase, N, Nth, p.interacting, p.precision, p.dt};
-    k_tile_gemm<<<dim3((Nth + 15) / 16, (N + 15) / 16, p.n_inst), WG, 0, st>>>(gv, ev);
+    if (p.n_inst >= TGW_MIN_INST) {
+        const int ntj = (Nth + 15) / 16;
+        k_tile_gemm_wide<DvEpi, false><<<dim3((N + 15) / 16, (ntj + TGW_T - 1) / TGW_T, p.n_inst), WG, 0, st>>>(gv, ev);
+    } else {
+        k_tile_gemm<<<dim3((Nth + 15) / 16, (N + 15) / 16, p.n_inst), WG, 0, st>>>(gv, ev);
+    }
     HIPCHK(ctx, hipGetLastError());
     return GGB_OK;
 }

@@ -204,6 +204,29 @@ def test_fp32_short_run(atom, N, R, lmax, nsteps):
     assert np.abs(out["norm"][:, 0, :] - ref["norm"]).max() < 1e-5
 
 
+def test_potential_panel_form_equals_small_tiles():
+    """From 8 instances on, the multipole moments and dV use the panel form of
+    the tile product (each wave keeps three tiles over one operand panel, with
+    the same products in the same order as the 16 x 16 form).  The potentials
+    of a 9-instance context equal those of each instance alone (the small-tile
+    form), bit for bit, at a grid with two tile groups and partial tiles
+    (He N=32 L_max=40: 81 multipoles and theta nodes)."""
+    _say("case: panel-form potential products, He N=32 Lmax=40, 9 instances vs alone")
+    S = AtomSystem("He", 32, 30.0, 40, 0.05)
+    n = 9
+    psis = [_random_state(S, seed=10 + k, scale=1.0 + 0.5 * k) for k in range(n)]
+    eng = _engine()
+    with eng.Propagator(S, n_inst=n) as prop:
+        prop.upload_orbitals(np.concatenate(psis, axis=1))
+        pot = prop.download_potential()
+    for k in range(n):
+        with eng.Propagator(S, n_inst=1) as prop:
+            prop.upload_orbitals(psis[k])
+            alone = prop.download_potential()
+        assert np.array_equal(pot[k], alone[0]), k
+    assert np.abs(pot[0] - pot[1]).max() > 1e-6  # the instances differ
+
+
 def test_graph_replay_matches_eager():
     """hipGraph replay (3 replays of 16 steps + 5 eager steps, eager work and
     synchronisations in between) reproduces plain launches bit for bit."""
