@@ -377,6 +377,9 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_bwd(const T *__restrict__ QT,
 // ---------------------------------------------------------------------------
 #define TM 64
 #define TN 128
+#ifndef KIN_DMA
+#define KIN_DMA 1  // FP32 tiled pass 1 through direct global-to-LDS copies (k_kin_fwd_dma)
+#endif
 #define TMP (TM + 16)
 #define TNP (TN + 16)
 
@@ -516,6 +519,143 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
     }
 }
 
+// FP32 tiled pass 1 with the tiles copied from global memory straight into
+// LDS (global_load_lds_dwordx4: no staging registers, no LDS stores), three
+// buffers and two K chunks in flight; one barrier per chunk.  It forms the
+// products of k_kin_fwd_tiled<float, 16> in the same order (same chunks, same
+// MFMA sequence, same two-level sum), so the stored Y is the same bit for bit.
+// Layouts, swizzled so that the operand reads of a wave hit 64 distinct banks:
+//   A chunk: 16 rows k x 64 a, element (k, m) at k 64 + ((m + 16 (k & 3)) & 63);
+//   B chunk: per complex column c (64 of them) a 128-byte block of 8 slots of
+//     (k, k+1) x (re, im); slot q = k/2 sits at position (q + c) & 7.
+// Rows a >= nk_max and columns beyond C load a valid in-range element instead
+// of zeros; they only reach outputs that are not stored.  Needs N % 16 == 0.
+typedef __attribute__((address_space(3))) void lds_void_t;
+#define KD_WAIT(n) __builtin_amdgcn_s_waitcnt(0xF70 | (n))  // vmcnt(n), other counters untouched
+__global__ __launch_bounds__(WG) void k_kin_fwd_dma(const float *__restrict__ Q,
+                                                    const double2 *__restrict__ kph,
+                                                    const int32_t *__restrict__ nkeep,
+                                                    const float2 *__restrict__ psi32, KinRef ref,
+                                                    float *__restrict__ Y, int N, int nk_max, int C,
+                                                    int n_orb) {
+    typedef mfma_t<float>::v4 v4;
+    // three buffers of (A chunk: 1024 floats, B chunk: 2048 floats)
+    __shared__ __align__(16) float lds[3 * 3072];
+    const tile3 tb = xcd_tile();  // (eigen tile, column tile, l)
+    const int l = tb.z;
+    const int nk = nkeep[l];
+    const int a0 = tb.x * TM;
+    if (a0 >= nk) return;  // block-uniform
+    const int n0 = tb.y * TN;
+    if (!kin_cols_active(ref, n0, TN, C, n_orb)) return;  // idle instances only
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    // this lane's copy sources: A rows 4 wave + lane / 16 (one instruction per
+    // chunk), B columns 16 wave + 8 j + lane / 8 (two instructions per chunk)
+    const int ka = 4 * wave + (lane >> 4);
+    const int ma = ((lane & 15) * 4 - 16 * (ka & 3)) & 63;
+    const float *Asrc = Q + ((size_t)l * N + ka) * nk_max + (a0 + ma < nk_max ? a0 + ma : 0);
+    const float2 *Bsrc[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int cl = 16 * wave + 8 * j + (lane >> 3);
+        const int q = ((lane & 7) - cl) & 7;
+        const int cg = (n0 >> 1) + cl;
+        Bsrc[j] = psi32 + ((size_t)l * C + (cg < C ? cg : 0)) * N + 2 * q;
+    }
+    auto copy = [&](int c, int buf) {
+        const int i0 = 16 * c;
+        float *As = lds + 3072 * buf, *Bs = As + 1024;
+        __builtin_amdgcn_global_load_lds((const void *)(Asrc + (size_t)i0 * nk_max), (lds_void_t *)(As + 256 * wave), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void *)(Bsrc[0] + i0), (lds_void_t *)(Bs + 512 * wave), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void *)(Bsrc[1] + i0), (lds_void_t *)(Bs + 512 * wave + 256), 16, 0, 0);
+    };
+    // operand offsets within a buffer, the same for every chunk
+    int oa0[4], oa1[4], ob[4][4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        const int kk = 4 * ks + kr;
+        oa0[ks] = kk * 64 + ((wm + lc + 16 * (kk & 3)) & 63);
+        oa1[ks] = kk * 64 + ((wm + 16 + lc + 16 * (kk & 3)) & 63);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int n = wn + 16 * y + lc, cc = n >> 1;
+            ob[ks][y] = 1024 + cc * 32 + (((kk >> 1) + cc) & 7) * 4 + (kk & 1) * 2 + (n & 1);
+        }
+    }
+    v4 acc[2][4], hi[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = (v4){0, 0, 0, 0}, hi[x][y] = (v4){0, 0, 0, 0};
+    const int nch = N / 16;
+    // chunk c from buffer buf; the copy of chunk c + 2 goes to buffer nb, the
+    // one of chunk c - 1, which every wave has finished with at the barrier
+    auto chunk = [&](int c, int buf, int nb) {
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's reads of the last chunk are done
+        if (c + 1 < nch)
+            KD_WAIT(3);  // this wave's copies of chunk c are done (chunk c + 1's may be in flight)
+        else
+            KD_WAIT(0);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (c + 2 < nch) copy(c + 2, nb);
+        const float *L = lds + 3072 * buf;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const float av0 = L[oa0[ks]];
+            const float av1 = L[oa1[ks]];
+            float b[4];
+#pragma unroll
+            for (int y = 0; y < 4; ++y) b[y] = L[ob[ks][y]];
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                acc[0][y] = mfma_t<float>::op(av0, b[y], acc[0][y]);
+                acc[1][y] = mfma_t<float>::op(av1, b[y], acc[1][y]);
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) hi[x][y] += acc[x][y], acc[x][y] = (v4){0, 0, 0, 0};
+    };
+    copy(0, 0);
+    if (nch > 1) copy(1, 1);
+    for (int c = 0; c < nch; c += 3) {
+        chunk(c, 0, 2);
+        if (c + 1 < nch) chunk(c + 1, 1, 0);
+        if (c + 2 < nch) chunk(c + 2, 2, 1);
+    }
+    const double2 *kphl = kph + (size_t)l * nk_max;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int col = n0 + wn + 16 * y + lc;
+            const int c = col >> 1, p = col & 1;
+            const int ab = a0 + wm + 16 * x;
+            float o[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = ab + mfma_t<float>::row(kr, r);
+                const double v = (double)hi[x][y][r];
+                const double vo = __shfl_xor(v, 1);
+                o[r] = kin_factor<float>(v, vo, p, kphl[a < nk ? a : 0]);
+            }
+            if (c < C) {
+                float *Yo = Y + (((size_t)l * C + c) * 2 + p) * nk_max;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int a = ab + mfma_t<float>::row(kr, r);
+                    if (a < nk) Yo[a] = o[r];
+                }
+            }
+        }
+    }
+}
+
 template <typename T, int TKT>
 __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
                                                       const int32_t *__restrict__ nkeep,
@@ -602,6 +742,127 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = i0 + wm + 16 * x + mfma_t<T>::row(kr, r);
+                if (i < N) {
+                    double v = (double)hi[x][y][r];
+                    if (rrow) v += rc * rrow[i];
+                    Xl[((size_t)c * N + i) * 2 + p] = v;
+                }
+            }
+        }
+    }
+}
+
+// FP32 tiled pass 2 with direct global-to-LDS copies, as k_kin_fwd_dma: the
+// products, chunks and sums of k_kin_bwd_tiled<float, 16> in the same order
+// (the same psi bit for bit).  Both operands are K-contiguous in memory (Q
+// rows i, Y rows of a column), so the chunk tiles are kept row-major by their
+// M or N index: element (row, k) at row 16 + 4 ((k / 4 + (row / 4)) & 3) + k % 4,
+// which puts the operand reads of a wave on 64 distinct banks.  Rows i >= N
+// and columns beyond C load row/column 0 and are not stored.
+__global__ __launch_bounds__(WG) void k_kin_bwd_dma(const float *__restrict__ Q,
+                                                    const int32_t *__restrict__ nkeep,
+                                                    const float *__restrict__ Y, KinRef ref,
+                                                    double *__restrict__ psi, int N, int nk_max,
+                                                    int C, int n_orb) {
+    typedef mfma_t<float>::v4 v4;
+    __shared__ __align__(16) float lds[3 * 3072];  // three buffers of (A: 64 x 16, B: 128 x 16)
+    const tile3 tb = xcd_tile();  // (radial tile, column tile, l)
+    const int l = tb.z;
+    const int nk = nkeep[l];
+    const int i0 = tb.x * TM;
+    const int n0 = tb.y * TN;
+    if (!kin_cols_active(ref, n0, TN, C, n_orb)) return;  // idle instances only
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int kr = lane >> 4, lc = lane & 15;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    // copy sources: A rows 16 wave + lane / 4 (one instruction), B columns
+    // 32 wave + 16 j + lane / 4 (two instructions); slot lane % 4 of a row holds
+    // the K quad (slot - row / 4) & 3
+    const int ra = 16 * wave + (lane >> 2);
+    const int qa = ((lane & 3) - (ra >> 2)) & 3;
+    const float *Asrc = Q + ((size_t)l * N + (i0 + ra < N ? i0 + ra : 0)) * nk_max + 4 * qa;
+    const float *Bsrc[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int rb = 32 * wave + 16 * j + (lane >> 2);
+        const int qb = ((lane & 3) - (rb >> 2)) & 3;
+        const int col = n0 + rb;
+        Bsrc[j] = Y + ((size_t)l * C * 2 + ((col >> 1) < C ? col : 0)) * nk_max + 4 * qb;
+    }
+    auto copy = [&](int c, int buf) {
+        const int k0 = 16 * c;
+        float *As = lds + 3072 * buf, *Bs = As + 1024;
+        __builtin_amdgcn_global_load_lds((const void *)(Asrc + k0), (lds_void_t *)(As + 256 * wave), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void *)(Bsrc[0] + k0), (lds_void_t *)(Bs + 512 * wave), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void *)(Bsrc[1] + k0), (lds_void_t *)(Bs + 512 * wave + 256), 16, 0, 0);
+    };
+    int oa0[4], oa1[4], ob[4][4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        const int m0 = wm + lc, m1 = wm + 16 + lc;
+        oa0[ks] = m0 * 16 + 4 * ((ks + (m0 >> 2)) & 3) + kr;
+        oa1[ks] = m1 * 16 + 4 * ((ks + (m1 >> 2)) & 3) + kr;
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int n = wn + 16 * y + lc;
+            ob[ks][y] = 1024 + n * 16 + 4 * ((ks + (n >> 2)) & 3) + kr;
+        }
+    }
+    v4 acc[2][4], hi[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = (v4){0, 0, 0, 0}, hi[x][y] = (v4){0, 0, 0, 0};
+    const int nch = (nk + 15) / 16;
+    auto chunk = [&](int c, int buf, int nb) {
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's reads of the last chunk are done
+        if (c + 1 < nch)
+            KD_WAIT(3);
+        else
+            KD_WAIT(0);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (c + 2 < nch) copy(c + 2, nb);
+        const float *L = lds + 3072 * buf;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const float av0 = L[oa0[ks]];
+            const float av1 = L[oa1[ks]];
+            float b[4];
+#pragma unroll
+            for (int y = 0; y < 4; ++y) b[y] = L[ob[ks][y]];
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                acc[0][y] = mfma_t<float>::op(av0, b[y], acc[0][y]);
+                acc[1][y] = mfma_t<float>::op(av1, b[y], acc[1][y]);
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) hi[x][y] += acc[x][y], acc[x][y] = (v4){0, 0, 0, 0};
+    };
+    if (nch > 0) copy(0, 0);
+    if (nch > 1) copy(1, 1);
+    for (int c = 0; c < nch; c += 3) {
+        chunk(c, 0, 2);
+        if (c + 1 < nch) chunk(c + 1, 1, 0);
+        if (c + 2 < nch) chunk(c + 2, 2, 1);
+    }
+    double *Xl = psi + (size_t)l * C * N * 2;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int col = n0 + wn + 16 * y + lc;
+            const int c = col >> 1, p = col & 1;
+            if (c >= C) continue;
+            double rc = 0.0;
+            const double *rrow = kin_ref_row(ref, l, c, p, N, n_orb, true, &rc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wm + 16 * x + mfma_t<float>::row(kr, r);
                 if (i < N) {
                     double v = (double)hi[x][y][r];
                     if (rrow) v += rc * rrow[i];
@@ -2693,12 +2954,18 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh, hipEvent_t e0 =
         if (pass == 0 && k32)
             kin_launch(k_kin_fwd_tiled<T, 32>, gf, WG, ctx->stream, e0, e1, Q,
                                   kph, nkeep, psi_in, psi32, ref, Y, N, nkm, C, no);
+        else if (pass == 0 && mfma_t<T>::mixed && KIN_DMA && N % 16 == 0)
+            kin_launch(k_kin_fwd_dma, gf, WG, ctx->stream, e0, e1, reinterpret_cast<const float *>(Q),
+                       kph, nkeep, psi32, ref, reinterpret_cast<float *>(Y), N, nkm, C, no);
         else if (pass == 0)
             kin_launch(k_kin_fwd_tiled<T, 16>, gf, WG, ctx->stream, e0, e1, Q,
                                   kph, nkeep, psi_in, psi32, ref, Y, N, nkm, C, no);
         else if (k32)
             kin_launch(k_kin_bwd_tiled<T, 32>, gb, WG, ctx->stream, e0, e1,
                                   (const T *)Q, nkeep, (const T *)Y, ref, psi_out, N, nkm, C, no);
+        else if (mfma_t<T>::mixed && KIN_DMA)
+            kin_launch(k_kin_bwd_dma, gb, WG, ctx->stream, e0, e1, reinterpret_cast<const float *>(Q), nkeep,
+                       reinterpret_cast<const float *>(Y), ref, psi_out, N, nkm, C, no);
         else
             kin_launch(k_kin_bwd_tiled<T, 16>, gb, WG, ctx->stream, e0, e1,
                                   (const T *)Q, nkeep, (const T *)Y, ref, psi_out, N, nkm, C, no);
