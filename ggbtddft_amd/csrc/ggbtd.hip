@@ -524,7 +524,8 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
 // buffers and two K chunks in flight; one barrier per chunk.  It forms the
 // products of k_kin_fwd_tiled<float, 16> in the same order (same chunks, same
 // MFMA sequence, same two-level sum), so the stored Y is the same bit for bit.
-// Layouts, swizzled so that the operand reads of a wave hit 64 distinct banks:
+// Layouts, swizzled so that the operand reads of each half-wave hit 32
+// distinct banks (the LDS serves 32 banks per half-wave):
 //   A chunk: 16 rows k x 64 a, element (k, m) at k 64 + ((m + 16 (k & 3)) & 63);
 //   B chunk: per complex column c (64 of them) a 128-byte block of 8 slots of
 //     (k, k+1) x (re, im); slot q = k/2 sits at position (q + c) & 7.
@@ -756,9 +757,11 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_tiled(const T *__restrict__ Q,
 // products, chunks and sums of k_kin_bwd_tiled<float, 16> in the same order
 // (the same psi bit for bit).  Both operands are K-contiguous in memory (Q
 // rows i, Y rows of a column), so the chunk tiles are kept row-major by their
-// M or N index: element (row, k) at row 16 + 4 ((k / 4 + (row / 4)) & 3) + k % 4,
-// which puts the operand reads of a wave on 64 distinct banks.  Rows i >= N
-// and columns beyond C load row/column 0 and are not stored.
+// M or N index: element (row, k) at row 16 + 4 ((k / 4 + (row / 4)) & 3) + k % 4.
+// The LDS serves 32 banks per half-wave, and the 16 lanes of a half-wave read
+// 16 different 64-byte rows: a 2-way bank conflict that no rotation within a
+// row removes.  Rows i >= N and columns beyond C load row/column 0 and are
+// not stored.
 __global__ __launch_bounds__(WG) void k_kin_bwd_dma(const float *__restrict__ Q,
                                                     const int32_t *__restrict__ nkeep,
                                                     const float *__restrict__ Y, KinRef ref,
