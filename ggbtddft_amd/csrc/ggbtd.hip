@@ -1179,6 +1179,9 @@ __global__ __launch_bounds__(WG) void k_kin_bwd_split(const float *__restrict__ 
 // max(ceil(h/16), ceil(ns/16) + ceil(na/16)) <= 8 waves.
 // ---------------------------------------------------------------------------
 #define ANG_MAXW 8
+#ifndef ANG_DMA
+#define ANG_DMA 1  // tables and l-space tile copied straight into LDS
+#endif
 #define ANG_MAX_NL 128  // largest N_l ggb_create accepts
 struct AngArgs {
     double *psi;
@@ -1372,21 +1375,47 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
     bool oku[NT];
 #pragma unroll
     for (int u = 0; u < NT; ++u) iu[u] = i0 + RT * u + ilc, oku[u] = iu[u] < g.N;
-    if (!GLB) {
-        // the block of one m is a multiple of 16 bytes (host pads nblk)
-        const int n16 = g.nblk * (int)sizeof(T) / 16;
-        const int4 *s4 = reinterpret_cast<const int4 *>(Bsrc);
-        int4 *d4 = reinterpret_cast<int4 *>(Tl);
-        for (int idx = t; idx < n16; idx += nthr) d4[idx] = s4[idx];
-    }
-    // load the l-space tile
-    for (int idx = t; idx < g.Nlx * R; idx += nthr) {
-        const int l = idx / R, il = idx % R;
-        const int ii = i0 + il;
-        double2 v = make_double2(0.0, 0.0);
-        if (l < g.Nl && ii < g.N) v = ((const double2 *)g.psi)[((size_t)l * g.C + c) * g.N + ii];
-        X[l * XS + 2 * il] = v.x;
-        X[l * XS + 2 * il + 1] = v.y;
+    if (ANG_DMA) {
+        // straight from global memory into LDS (global_load_lds_dwordx4: 16 bytes
+        // per lane, 1 KB per wave instruction, no staging registers)
+        const int nw = nthr >> 6;
+        if (!GLB) {
+            // the block of one m is a multiple of 16 bytes (host pads nblk)
+            const int n16 = g.nblk * (int)sizeof(T) / 16;
+            for (int b = 64 * wave; b < n16; b += 64 * nw)
+                if (b + lane < n16)
+                    __builtin_amdgcn_global_load_lds((const void *)(reinterpret_cast<const int4 *>(Bsrc) + b + lane),
+                                                     (lds_void_t *)(reinterpret_cast<int4 *>(Tl) + b), 16, 0, 0);
+        }
+        // rows l < N_l of the l-space tile: a row is the R nodes i0.. of psi[l][c]
+        // (R x 16 bytes, contiguous in both), 64 / R rows per instruction; nodes
+        // past N load node N - 1 (their results are not stored)
+        constexpr int RPI = 64 / R;
+        const int ii = min(i0 + lane % R, g.N - 1);
+        for (int l0 = RPI * wave; l0 < g.Nl; l0 += RPI * nw) {
+            const int l = l0 + lane / R;
+            if (l < g.Nl)
+                __builtin_amdgcn_global_load_lds((const void *)((const double2 *)g.psi + ((size_t)l * g.C + c) * g.N + ii),
+                                                 (lds_void_t *)(X + l0 * XS), 16, 0, 0);
+        }
+        for (int idx = t; idx < (g.Nlx - g.Nl) * XS; idx += nthr) X[g.Nl * XS + idx] = 0.0;
+    } else {
+        if (!GLB) {
+            // the block of one m is a multiple of 16 bytes (host pads nblk)
+            const int n16 = g.nblk * (int)sizeof(T) / 16;
+            const int4 *s4 = reinterpret_cast<const int4 *>(Bsrc);
+            int4 *d4 = reinterpret_cast<int4 *>(Tl);
+            for (int idx = t; idx < n16; idx += nthr) d4[idx] = s4[idx];
+        }
+        // load the l-space tile
+        for (int idx = t; idx < g.Nlx * R; idx += nthr) {
+            const int l = idx / R, il = idx % R;
+            const int ii = i0 + il;
+            double2 v = make_double2(0.0, 0.0);
+            if (l < g.Nl && ii < g.N) v = ((const double2 *)g.psi)[((size_t)l * g.C + c) * g.N + ii];
+            X[l * XS + 2 * il] = v.x;
+            X[l * XS + 2 * il + 1] = v.y;
+        }
     }
     if (MODE != 2 && t < R) {
         const int i = i0 + t;
