@@ -93,11 +93,12 @@ def test_substeps(atom, N, R, lmax):
 
 @pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30),
                                                    ("Kr", 64, 40.0, 8, 20), ("Xe", 64, 40.0, 10, 12),
-                                                   ("He", 32, 30.0, 96, 10)])
+                                                   ("He", 32, 30.0, 96, 10), ("Ne", 44, 40.0, 8, 20)])
 def test_short_run(atom, N, R, lmax, nsteps):
     # Kr: 13 orbitals with |m| up to 2; Xe: 4d shells, three |m| groups; He at
     # Lmax = 96: the FP64 angular tables exceed the LDS budget and are read
-    # from global memory
+    # from global memory; Ne at N = 44: the last 8-node tile of the angular
+    # kernel holds 4 nodes past N (copied in as node N-1, not stored)
     _say("case: short run", atom, N, lmax, nsteps)
     dt = 0.05
     S, O = _pair(atom, N, R, lmax, dt)
@@ -182,10 +183,11 @@ def test_batched_instances_match_alone(n_inst):
 
 
 @pytest.mark.parametrize("atom,N,R,lmax,nsteps", [("He", 64, 40.0, 8, 60), ("Ar", 64, 40.0, 8, 30),
-                                                   ("He", 32, 30.0, 96, 10)])
+                                                   ("He", 32, 30.0, 96, 10), ("Ne", 40, 40.0, 8, 20)])
 def test_fp32_short_run(atom, N, R, lmax, nsteps):
     """Mixed precision (FP32 tables/transforms/v_xc, FP64 state and sums)
-    against the FP64 oracle: FP32-level agreement."""
+    against the FP64 oracle: FP32-level agreement.  Ne at N = 40: the last
+    16-node tile of the FP32 angular kernel holds 8 nodes past N."""
     _say("case: fp32 short run", atom, N, lmax, nsteps)
     dt = 0.05
     S, O = _pair(atom, N, R, lmax, dt)
@@ -254,14 +256,15 @@ def test_graph_replay_matches_eager():
     assert np.array_equal(psi, psi_ref)
 
 
-@pytest.mark.parametrize("precision,N", [(0, 48), (1, 48), (1, 40)])
+@pytest.mark.parametrize("precision,N", [(0, 48), (1, 48), (1, 40), (0, 44)])
 def test_kinetic_variants_agree_and_autotune(precision, N):
     """Every kinetic kernel variant gives the same run (to rounding: they sum
     K in different orders); the autotuner picks one of them and leaves the
     orbitals as they were.  N = 40 is not a multiple of the 16-row chunks of
     the FP32 tiled pass 1 through direct global-to-LDS copies (it runs the
     register-staged form there) and leaves 24 clamped rows in the last radial
-    tile of pass 2."""
+    tile of pass 2.  N = 44 leaves 4 clamped nodes in the last radial tile of
+    the FP64 angular kernel, whose l-space tile is copied straight into LDS."""
     _say("case: kinetic variants, Ne N=%d Lmax=6, 3 instances, precision %d" % (N, precision))
     dt = 0.05
     S = AtomSystem("Ne", N, 30.0, 6, dt)
