@@ -217,14 +217,14 @@ static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, in
 // ---------------------------------------------------------------------------
 // Kinetic propagator, pass 1: Y[l][c][p][a] = (e^{-i lam_a dt} sum_i Q[l][i][a] psi[l][c][i])_p
 // M = eigen index a, N = real column (c, p = re/im), K = radial node i.
-// One work-group computes one 16 x 16 output tile; its 4 waves take the four
+// One work-group computes 16 TA x 16 outputs; its 4 waves take the four
 // quarters of K (more loads in flight per CU than one wave per tile), then the
 // partial tiles are added in fixed order through LDS.  Y is planar in (re, im)
 // so that pass 2 reads contiguous runs of a.  T is the compute type (tables
 // and Y in T, state in FP64, phase in FP64).
 // ---------------------------------------------------------------------------
 #ifndef KIN_KU
-#define KIN_KU 8
+#define KIN_KU 4
 #endif
 #define KU KIN_KU
 #ifndef KIN_NW
@@ -233,7 +233,19 @@ static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, in
 // waves 0..3 finish accumulator rows 0..3 of every tile after the K split
 static_assert(KIN_NW >= 4 && KIN_NW <= 16, "KIN_NW: 4 to 16 waves per narrow work-group");
 #define KIN_WG (64 * KIN_NW)
-template <typename T>
+// Panel form: one work-group takes the 16 TA eigen rows a0 .. a0 + 16 TA - 1 as
+// TA tiles with interleaved rows (row ar of tile t is a = a0 + TA ar + t).  A
+// lane reads its TA table values of a K row as one vector, the 16 lanes of the
+// row read one run of 16 TA values, and each state load feeds TA products.
+// Every tile keeps the K split, the product order and the fixed-order LDS sum
+// of one tile per work-group, so the result does not depend on TA.  TA = 2
+// takes an argon FP64 pass from 25.4 to 22.9 us (FP32 22.8 to 20.7); TA = 4
+// is no faster in FP64 and slower in FP32.
+#ifndef KIN_TA
+#define KIN_TA 2
+#endif
+static_assert(KIN_TA == 1 || KIN_TA == 2 || KIN_TA == 4, "KIN_TA: 1, 2 or 4 (a divisor of 16)");
+template <typename T, int TA>
 __global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 const double2 *__restrict__ kph,
                                                 const int32_t *__restrict__ nkeep,
@@ -241,11 +253,12 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 T *__restrict__ Y, int N, int nk_max, int C,
                                                 int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
-    __shared__ double red[KIN_NW][64][4];
-    const tile3 tb = xcd_tile();  // (eigen tile, column tile, l): tiles of one l share psi
+    typedef T vta __attribute__((ext_vector_type(TA)));
+    __shared__ double red[KIN_NW][TA][64][4];
+    const tile3 tb = xcd_tile();  // (eigen panel, column tile, l): panels of one l share psi
     const int l = tb.z;
     const int nk = nkeep[l];
-    const int a0 = tb.x * 16;
+    const int a0 = tb.x * 16 * TA;
     if (a0 >= nk) return;  // block-uniform
     if (!kin_cols_active(ref, tb.y * 16, 16, C, n_orb)) return;  // idle instances only
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -253,41 +266,57 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
     const int col = tb.y * 16 + lc;
     const int c = col >> 1, p = col & 1;
     const bool colok = c < C;
-    const T *Ql = Q + (size_t)l * N * nk_max + a0 + lc;
+    const T *Ql = Q + (size_t)l * N * nk_max + a0 + TA * lc;
+    const bool lv = a0 + TA * lc < nk;  // interleaved rows: this lane's TA rows hold some a < nkeep
     const double *Xl = psi + ((size_t)l * C + (colok ? c : 0)) * N * 2 + p;
     const int kq = (((N + KIN_NW - 1) / KIN_NW + 3) / 4) * 4;  // K share of a wave, multiple of 4
     const int kb = wave * kq, ke = min(N, kb + kq);
     double rc = 0.0;
     const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, false, &rc) : nullptr;
-    v4 acc = {0, 0, 0, 0};
+    v4 acc[TA];
+#pragma unroll
+    for (int t = 0; t < TA; ++t) acc[t] = v4{0, 0, 0, 0};
     for (int k0 = kb; k0 < ke; k0 += 4 * KU) {
-        T av[KU], bv[KU];
+        T av[KU][TA], bv[KU];
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
             const int i = k0 + 4 * u + kr;
             const bool ok = i < ke;
-            av[u] = ok ? Ql[(size_t)i * nk_max] : (T)0;
+#pragma unroll
+            for (int t = 0; t < TA; ++t) av[u][t] = (T)0;
+            if (ok && lv) {
+                const vta q = *reinterpret_cast<const vta *>(Ql + (size_t)i * nk_max);
+#pragma unroll
+                for (int t = 0; t < TA; ++t) av[u][t] = q[t];
+            }
             double x = (ok && colok) ? Xl[(size_t)i * 2] : 0.0;
             if (rrow && ok) x -= rc * rrow[i];
             bv[u] = (T)x;
         }
 #pragma unroll
-        for (int u = 0; u < KU; ++u) acc = mfma_t<T>::op(av[u], bv[u], acc);
+        for (int u = 0; u < KU; ++u)
+#pragma unroll
+            for (int t = 0; t < TA; ++t) acc[t] = mfma_t<T>::op(av[u][t], bv[u], acc[t]);
     }
     // partial tiles through LDS (wave 0's in the last slot); wave w < 4 then finishes
-    // accumulator row r = w, so the four epilogues run side by side
+    // accumulator row r = w of every tile, so the four epilogues run side by side
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : KIN_NW - 1][lane][r] = (double)acc[r];
+    for (int t = 0; t < TA; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : KIN_NW - 1][t][lane][r] = (double)acc[t][r];
     __syncthreads();
     if (wave >= 4) return;
     const int r = wave;
-    double v = red[KIN_NW - 1][lane][r];
-    for (int w = 0; w < KIN_NW - 1; ++w) v += red[w][lane][r];
-    const double vo = __shfl_xor(v, 1);
     const int ar = mfma_t<T>::row(kr, r);
-    const int a = a0 + ar;
-    const T o = kin_factor<T>(v, vo, p, kph[(size_t)l * nk_max + (a < nk ? a : 0)]);
-    if (colok && a < nk) Y[(((size_t)l * C + c) * 2 + p) * nk_max + a] = o;
+#pragma unroll
+    for (int t = 0; t < TA; ++t) {
+        double v = red[KIN_NW - 1][t][lane][r];
+        for (int w = 0; w < KIN_NW - 1; ++w) v += red[w][t][lane][r];
+        const double vo = __shfl_xor(v, 1);
+        const int a = a0 + TA * ar + t;
+        const T o = kin_factor<T>(v, vo, p, kph[(size_t)l * nk_max + (a < nk ? a : 0)]);
+        if (colok && a < nk) Y[(((size_t)l * C + c) * 2 + p) * nk_max + a] = o;
+    }
 }
 
 // element (a, i) of block l of the quad-interleaved transposed table of the
@@ -3017,7 +3046,8 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh, hipEvent_t e0 =
     const int N = p.n_r, nkm = p.nk_max, C = ctx->C, no = p.n_orb;
     if (pass == 0) {
         const double *psi_in = ctx->psi;
-        kin_launch(k_kin_fwd<T>, dim3(p.nk_max / 16, ncol, p.n_l), KIN_WG, ctx->stream, e0, e1, Q, kph, nkeep,
+        kin_launch(k_kin_fwd<T, KIN_TA>, dim3((p.nk_max / 16 + KIN_TA - 1) / KIN_TA, ncol, p.n_l), KIN_WG, ctx->stream,
+                   e0, e1, Q, kph, nkeep,
                    psi_in, ref, Y, N, nkm, C, no);
     } else {
         kin_launch(k_kin_bwd<T>, dim3((p.n_r + 15) / 16, ncol, p.n_l), KIN_WG, ctx->stream, e0, e1,
