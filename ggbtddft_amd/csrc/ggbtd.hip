@@ -217,22 +217,32 @@ static __device__ __forceinline__ const double *kin_ref_row(const KinRef &kr, in
 // ---------------------------------------------------------------------------
 // Kinetic propagator, pass 1: Y[l][c][p][a] = (e^{-i lam_a dt} sum_i Q[l][i][a] psi[l][c][i])_p
 // M = eigen index a, N = real column (c, p = re/im), K = radial node i.
-// One work-group computes 16 TA x 16 outputs; its 4 waves take the four
-// quarters of K (more loads in flight per CU than one wave per tile), then the
-// partial tiles are added in fixed order through LDS.  Y is planar in (re, im)
+// One work-group computes 16 TA x 16 outputs; its NW waves take equal shares
+// of K (more loads in flight per CU than one wave per tile), then the partial
+// tiles are added in fixed order through LDS.  Y is planar in (re, im)
 // so that pass 2 reads contiguous runs of a.  T is the compute type (tables
 // and Y in T, state in FP64, phase in FP64).
 // ---------------------------------------------------------------------------
 #ifndef KIN_KU
-#define KIN_KU 4
+#define KIN_KU 2
 #endif
 #define KU KIN_KU
+// waves per work-group of the narrow passes (K split over them): 4 in FP64, 8
+// in the mixed form, whose FP32 pass 1 takes 17.8 instead of 19.7 us at argon
+// size with 8 (FP64: 22.6 against 22.1 us)
 #ifndef KIN_NW
-#define KIN_NW 4  // waves per work-group of the narrow passes (K split over them), >= 4
+#define KIN_NW 4
+#endif
+#ifndef KIN_NW32
+#define KIN_NW32 8
 #endif
 // waves 0..3 finish accumulator rows 0..3 of every tile after the K split
-static_assert(KIN_NW >= 4 && KIN_NW <= 16, "KIN_NW: 4 to 16 waves per narrow work-group");
-#define KIN_WG (64 * KIN_NW)
+static_assert(KIN_NW >= 4 && KIN_NW <= 16 && KIN_NW32 >= 4 && KIN_NW32 <= 16, "4 to 16 waves per narrow work-group");
+template <typename T>
+struct kin_nw {
+    static constexpr int value = mfma_t<T>::mixed ? KIN_NW32 : KIN_NW;
+};
+#define KIN_WG_MAX (64 * (KIN_NW > KIN_NW32 ? KIN_NW : KIN_NW32))
 // Panel form: one work-group takes the 16 TA eigen rows a0 .. a0 + 16 TA - 1 as
 // TA tiles with interleaved rows (row ar of tile t is a = a0 + TA ar + t).  A
 // lane reads its TA table values of a K row as one vector, the 16 lanes of the
@@ -246,7 +256,7 @@ static_assert(KIN_NW >= 4 && KIN_NW <= 16, "KIN_NW: 4 to 16 waves per narrow wor
 #endif
 static_assert(KIN_TA == 1 || KIN_TA == 2 || KIN_TA == 4, "KIN_TA: 1, 2 or 4 (a divisor of 16)");
 template <typename T, int TA>
-__global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
+__global__ __launch_bounds__(KIN_WG_MAX) void k_kin_fwd(const T *__restrict__ Q,
                                                 const double2 *__restrict__ kph,
                                                 const int32_t *__restrict__ nkeep,
                                                 const double *__restrict__ psi, KinRef ref,
@@ -254,7 +264,8 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
                                                 int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
     typedef T vta __attribute__((ext_vector_type(TA)));
-    __shared__ double red[KIN_NW][TA][64][4];
+    constexpr int NW = kin_nw<T>::value;
+    __shared__ double red[NW][TA][64][4];
     const tile3 tb = xcd_tile();  // (eigen panel, column tile, l): panels of one l share psi
     const int l = tb.z;
     const int nk = nkeep[l];
@@ -269,7 +280,7 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
     const T *Ql = Q + (size_t)l * N * nk_max + a0 + TA * lc;
     const bool lv = a0 + TA * lc < nk;  // interleaved rows: this lane's TA rows hold some a < nkeep
     const double *Xl = psi + ((size_t)l * C + (colok ? c : 0)) * N * 2 + p;
-    const int kq = (((N + KIN_NW - 1) / KIN_NW + 3) / 4) * 4;  // K share of a wave, multiple of 4
+    const int kq = (((N + NW - 1) / NW + 3) / 4) * 4;  // K share of a wave, multiple of 4
     const int kb = wave * kq, ke = min(N, kb + kq);
     double rc = 0.0;
     const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, false, &rc) : nullptr;
@@ -303,15 +314,15 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_fwd(const T *__restrict__ Q,
 #pragma unroll
     for (int t = 0; t < TA; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : KIN_NW - 1][t][lane][r] = (double)acc[t][r];
+        for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : NW - 1][t][lane][r] = (double)acc[t][r];
     __syncthreads();
     if (wave >= 4) return;
     const int r = wave;
     const int ar = mfma_t<T>::row(kr, r);
 #pragma unroll
     for (int t = 0; t < TA; ++t) {
-        double v = red[KIN_NW - 1][t][lane][r];
-        for (int w = 0; w < KIN_NW - 1; ++w) v += red[w][t][lane][r];
+        double v = red[NW - 1][t][lane][r];
+        for (int w = 0; w < NW - 1; ++w) v += red[w][t][lane][r];
         const double vo = __shfl_xor(v, 1);
         const int a = a0 + TA * ar + t;
         const T o = kin_factor<T>(v, vo, p, kph[(size_t)l * nk_max + (a < nk ? a : 0)]);
@@ -326,7 +337,7 @@ static __host__ __device__ __forceinline__ size_t kin_qt_index(size_t l, size_t 
 }
 
 // Kinetic propagator, pass 2: psi[l][c][i] = sum_a QT[l][a][i] Y[l][c][a].
-// One work-group per 16 x 16 output tile, K = a split over the 4 waves as in
+// One work-group per 16 x 16 output tile, K = a split over the NW waves as in
 // pass 1.  Within a wave the K order is permuted (lane kr of MFMA j in chunk u
 // takes a = k0 + 16u + 4kr + j) so that each lane reads 4 consecutive a of Q
 // and of planar Y as one vector.  Q and Y are zero beyond nkeep up to nk_max
@@ -338,13 +349,14 @@ static __host__ __device__ __forceinline__ size_t kin_qt_index(size_t l, size_t 
 #endif
 #define KU2 KIN_KU2
 template <typename T>
-__global__ __launch_bounds__(KIN_WG) void k_kin_bwd(const T *__restrict__ QT,
+__global__ __launch_bounds__(KIN_WG_MAX) void k_kin_bwd(const T *__restrict__ QT,
                                                 const int32_t *__restrict__ nkeep,
                                                 const T *__restrict__ Y, KinRef ref,
                                                 double *__restrict__ psi, int N, int nk_max,
                                                 int C, int n_orb) {
     typedef typename mfma_t<T>::v4 v4;
-    __shared__ double red[KIN_NW][64][4];
+    constexpr int NW = kin_nw<T>::value;
+    __shared__ double red[NW][64][4];
     const tile3 tb = xcd_tile();  // (radial tile, column tile, l): tiles of one l share Y
     const int l = tb.z;
     const int nk = nkeep[l];
@@ -364,7 +376,7 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_bwd(const T *__restrict__ QT,
     // 24.2 us per FP64 argon pass (13.7 instead of 15.3 in FP32).
     const T *Qc = QT + kin_qt_index(l, im_ < N ? im_ : 0, 4 * kr, N, nk_max);
     const T *Yr = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max + 4 * kr;
-    const int kq = (((kend + 15) / 16 + KIN_NW - 1) / KIN_NW) * 16;  // K share of a wave, multiple of 16
+    const int kq = (((kend + 15) / 16 + NW - 1) / NW) * 16;  // K share of a wave, multiple of 16
     const int kb = wave * kq, ke = min(kend, kb + kq);
     v4 acc = {0, 0, 0, 0};
     for (int k0 = kb; k0 < ke; k0 += 16 * KU2) {
@@ -387,7 +399,7 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_bwd(const T *__restrict__ QT,
     // partial tiles through LDS (wave 0's in the last slot); wave w < 4 then finishes
     // accumulator row r = w, so the four epilogues run side by side
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : KIN_NW - 1][lane][r] = (double)acc[r];
+    for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : NW - 1][lane][r] = (double)acc[r];
     __syncthreads();
     if (wave >= 4) return;
     double *Xl = psi + (size_t)l * C * N * 2;
@@ -395,8 +407,8 @@ __global__ __launch_bounds__(KIN_WG) void k_kin_bwd(const T *__restrict__ QT,
     const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, true, &rc) : nullptr;
     const int r = wave;
     const int i = i0 + mfma_t<T>::row(kr, r);
-    double v = red[KIN_NW - 1][lane][r];
-    for (int w = 0; w < KIN_NW - 1; ++w) v += red[w][lane][r];
+    double v = red[NW - 1][lane][r];
+    for (int w = 0; w < NW - 1; ++w) v += red[w][lane][r];
     if (colok && i < N) {
         if (rrow) v += rc * rrow[i];
         Xl[((size_t)c * N + i) * 2 + p] = v;
@@ -3046,11 +3058,12 @@ static void launch_kinetic_t(ggb_ctx *ctx, int pass, bool fresh, hipEvent_t e0 =
     const int N = p.n_r, nkm = p.nk_max, C = ctx->C, no = p.n_orb;
     if (pass == 0) {
         const double *psi_in = ctx->psi;
-        kin_launch(k_kin_fwd<T, KIN_TA>, dim3((p.nk_max / 16 + KIN_TA - 1) / KIN_TA, ncol, p.n_l), KIN_WG, ctx->stream,
+        kin_launch(k_kin_fwd<T, KIN_TA>, dim3((p.nk_max / 16 + KIN_TA - 1) / KIN_TA, ncol, p.n_l), 64 * kin_nw<T>::value,
+                   ctx->stream,
                    e0, e1, Q, kph, nkeep,
                    psi_in, ref, Y, N, nkm, C, no);
     } else {
-        kin_launch(k_kin_bwd<T>, dim3((p.n_r + 15) / 16, ncol, p.n_l), KIN_WG, ctx->stream, e0, e1,
+        kin_launch(k_kin_bwd<T>, dim3((p.n_r + 15) / 16, ncol, p.n_l), 64 * kin_nw<T>::value, ctx->stream, e0, e1,
                    reinterpret_cast<const T *>(ctx->QT), nkeep, (const T *)Y, ref, ctx->psi, N, nkm,
                    C, no);
     }
