@@ -100,6 +100,19 @@ static __device__ __forceinline__ int xcd_linear() {
     const int q = T / 8, r = T % 8, x = p % 8;
     return x * q + min(x, r) + p / 8;
 }
+// the value of lane L ^ 1 (the re/im partner of a real column pair) through
+// a DPP quad permutation [1,0,3,2] in the VALU, where __shfl_xor takes an LDS
+// round trip (ds_bpermute) per 32 bits; the same value.  Every lane of the
+// wave must be active.
+static __device__ __forceinline__ float lane_xor1(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+static __device__ __forceinline__ double lane_xor1(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0xB1, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 struct tile3 {
     int x, y, z;
 };
@@ -323,7 +336,7 @@ __global__ __launch_bounds__(KIN_WG_MAX) void k_kin_fwd(const T *__restrict__ Q,
     for (int t = 0; t < TA; ++t) {
         double v = red[NW - 1][t][lane][r];
         for (int w = 0; w < NW - 1; ++w) v += red[w][t][lane][r];
-        const double vo = __shfl_xor(v, 1);
+        const double vo = lane_xor1(v);
         const int a = a0 + TA * ar + t;
         const T o = kin_factor<T>(v, vo, p, kph[(size_t)l * nk_max + (a < nk ? a : 0)]);
         if (colok && a < nk) Y[(((size_t)l * C + c) * 2 + p) * nk_max + a] = o;
@@ -553,7 +566,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_tiled(const T *__restrict__ Q,
             for (int r = 0; r < 4; ++r) {
                 const int a = ab + mfma_t<T>::row(kr, r);
                 const double v = (double)hi[x][y][r];
-                const double vo = __shfl_xor(v, 1);
+                const double vo = lane_xor1(v);
                 o[r] = kin_factor<T>(v, vo, p, kphl[a < nk ? a : 0]);
             }
             if (c < C) {
@@ -691,7 +704,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_dma(const float *__restrict__ Q,
             for (int r = 0; r < 4; ++r) {
                 const int a = ab + mfma_t<float>::row(kr, r);
                 const double v = (double)hi[x][y][r];
-                const double vo = __shfl_xor(v, 1);
+                const double vo = lane_xor1(v);
                 o[r] = kin_factor<float>(v, vo, p, kphl[a < nk ? a : 0]);
             }
             if (c < C) {
@@ -1075,7 +1088,7 @@ __global__ __launch_bounds__(WG) void k_kin_fwd_split(const float *__restrict__ 
             for (int r = 0; r < 4; ++r) {
                 const int a = ab + mfma_t<float>::row(kr, r);
                 const double v = hi[x][y][r];
-                const double vo = __shfl_xor(v, 1);
+                const double vo = lane_xor1(v);
                 o[r] = kin_factor<double>(v, vo, p, kphl[a < nk ? a : 0]);
             }
             if (c < C) {
@@ -1525,7 +1538,7 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                     const int j = wave * 16 + mfma_t<T>::row(kr, r);
                     const int jm = g.Nth - 1 - j;
                     const T pa = e[u][r] + o[u][r], pb = e[u][r] - o[u][r];
-                    const T pao = __shfl_xor(pa, 1), pbo = __shfl_xor(pb, 1);
+                    const T pao = lane_xor1(pa), pbo = lane_xor1(pb);
                     T sv = (T)0, dv = (T)0;
                     if (j < g.h && oku[u]) {
                         // component p of f (x + i y): x fx - y fy (p = 0), y fx + x fy (p = 1);
@@ -1636,9 +1649,9 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                 const int jm = g.Nth - 1 - j;
                 const double ed = (double)e[u][r], od = (double)o[u][r];
                 const double va = ed + od, vb = ed - od;
-                const double s2a = va * va + __shfl_xor(va * va, 1);
-                const double s2b = vb * vb + __shfl_xor(vb * vb, 1);
-                const double eo = 4.0 * (ed * od + __shfl_xor(ed * od, 1));  // |Psi_j|^2 - |Psi_j'|^2
+                const double s2a = va * va + lane_xor1(va * va);
+                const double s2b = vb * vb + lane_xor1(vb * vb);
+                const double eo = 4.0 * (ed * od + lane_xor1(ed * od));  // |Psi_j|^2 - |Psi_j'|^2
                 if (p == 0 && oku[u] && j < g.h) {
                     dens[j] = (T)s2a;
                     if (jm != j) {
