@@ -1254,7 +1254,8 @@ struct AngArgs {
     const double *r;
     const double *chi;     // [Nth]
     const double *E;       // [n_inst][n_field]
-    void *dens;            // [C][N][Nth] |Psi|^2 in the compute type
+    void *dens;            // [C][N][2][hq] |Psi|^2 in the compute type, folded: [..][0][j] is
+                           // node j < h, [..][1][j] its mirror N_theta - 1 - j
     double *obs_part;      // [C][ntiles][3]
     const int *cur;        // graph replay: field step = (cur[0] + step) mod n_field
     float2 *psi32;         // mode 0, FP32 tiled kinetic: FP32 copy of the stored state
@@ -1618,6 +1619,19 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         __syncthreads();
     }
 
+    // the density phase's r_i and chi_j, loaded ahead of the state stores: one
+    // vector-memory counter covers loads and stores, so a load issued after the
+    // stores would also wait for all of them
+    double rdn[NT], cdn[4];
+    if (MODE != 0) {
+#pragma unroll
+        for (int u = 0; u < NT; ++u) rdn[u] = oku[u] ? g.r[iu[u]] : 1.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = wave * 16 + mfma_t<T>::row(kr, r);
+            cdn[r] = g.chi[j < g.h ? j : 0];
+        }
+    }
     if (MODE != 2) {
         const int lref = (MODE == 0 && g.psi32 && g.l0) ? g.l0[korb] : -1;
         for (int idx = t; idx < g.Nl * R; idx += nthr) {
@@ -1640,9 +1654,11 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         ang_fwd<T, NT>(Bs, Ba, X, g, pr, m, wave, e, o);
 #pragma unroll
         for (int u = 0; u < NT; ++u) {
-            const double ri = oku[u] ? g.r[iu[u]] : 1.0;
+            const double ri = rdn[u];
             const double ir2 = 1.0 / (ri * ri);
-            T *dens = reinterpret_cast<T *>(g.dens) + ((size_t)c * g.N + (oku[u] ? iu[u] : 0)) * g.Nth;
+            // folded |Psi|^2 rows: lanes p = 0 store the nodes j, lanes p = 1 their mirrors
+            T *dens = reinterpret_cast<T *>(g.dens) + (((size_t)c * g.N + (oku[u] ? iu[u] : 0)) * 2 + p) * g.hq;
+            T dq[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int j = wave * 16 + mfma_t<T>::row(kr, r);
@@ -1652,16 +1668,34 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
                 const double s2a = va * va + lane_xor1(va * va);
                 const double s2b = vb * vb + lane_xor1(vb * vb);
                 const double eo = 4.0 * (ed * od + lane_xor1(ed * od));  // |Psi_j|^2 - |Psi_j'|^2
+                // lane p = 1 takes its partner's s2b, so that the mirror value is the
+                // one lane p = 0 forms (a contracted multiply-add is not symmetric)
+                const double s2bp = lane_xor1(s2b);
+                dq[r] = (T)(p ? s2bp : s2a);
                 if (p == 0 && oku[u] && j < g.h) {
-                    dens[j] = (T)s2a;
                     if (jm != j) {
-                        const double ca = g.chi[j];
-                        dens[jm] = (T)s2b;
+                        const double ca = cdn[r];
                         sd += ri * ca * eo;
                         sa += ca * eo * ir2;
                         sn += s2a + s2b;
                     } else {
                         sn += s2a;  // middle node, chi = 0
+                    }
+                }
+            }
+            // rows h .. hq - 1 hold values that k_rho never reads, and so does the
+            // mirror half at the middle node of an odd N_theta
+            if (oku[u]) {
+                if constexpr (mfma_t<T>::mixed) {
+                    // this lane's four items are the consecutive rows 4 kr .. 4 kr + 3
+                    // of its tile: one 16-byte store (hq is a multiple of 4)
+                    const int j0 = wave * 16 + 4 * kr;
+                    if (j0 < g.hq) *reinterpret_cast<float4 *>(dens + j0) = make_float4(dq[0], dq[1], dq[2], dq[3]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = wave * 16 + mfma_t<T>::row(kr, r);
+                        if (j < g.h) dens[j] = dq[r];
                     }
                 }
             }
@@ -1678,7 +1712,12 @@ __global__ __launch_bounds__(64 * ANG_MAXW) void k_angular(AngArgs g) {
         red[1][wave] = sa;
         red[2][wave] = sn;
     }
-    __syncthreads();
+    // barrier over LDS only: the state and |Psi|^2 stores above may still be in
+    // flight (a fenced barrier would wait for every one of them to complete)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's partial sums are in LDS
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
     if (t < 3) {
         double s = 0.0;
         for (int w = 0; w < nthr / 64; ++w) s += red[t][w];
@@ -1798,7 +1837,7 @@ __global__ __launch_bounds__(WG) void k_rho(const T *__restrict__ dens,
                                             const double *__restrict__ wr,
                                             const double *__restrict__ omega,
                                             double *__restrict__ rho_u, double *__restrict__ vxc,
-                                            int N, int Nth, int n_orb, ObsArgs ob,
+                                            int N, int Nth, int h, int hq, int n_orb, ObsArgs ob,
                                             const int32_t *__restrict__ active) {
     // the blocks past the (r, theta) grid (instance row 0) reduce the observable
     // partials of the step, one (column, observable) pair per wave
@@ -1814,19 +1853,21 @@ __global__ __launch_bounds__(WG) void k_rho(const T *__restrict__ dens,
     const int idx = (int)blockIdx.x * WG + (int)threadIdx.x;
     if (idx >= npt) return;
     const int i = idx / Nth, j = idx - i * Nth;
+    // the folded |Psi|^2 rows: node j < h in the first half, its mirror in the second
+    const size_t nd = (size_t)N * 2 * hq;  // one orbital column
+    const T *dk = dens + (size_t)inst * n_orb * nd + ((size_t)i * 2 + (j < h ? 0 : 1)) * hq + (j < h ? j : Nth - 1 - j);
     // four orbitals' loads in flight at a time, summed in orbital order
-    const T *dk = dens + (size_t)inst * n_orb * npt + idx;
     double ru = 0.0;
     int k = 0;
     for (; k + 4 <= n_orb; k += 4) {
-        const T d0 = dk[(size_t)k * npt], d1 = dk[(size_t)(k + 1) * npt];
-        const T d2 = dk[(size_t)(k + 2) * npt], d3 = dk[(size_t)(k + 3) * npt];
+        const T d0 = dk[k * nd], d1 = dk[(k + 1) * nd];
+        const T d2 = dk[(k + 2) * nd], d3 = dk[(k + 3) * nd];
         ru += occ[k] * d0;
         ru += occ[k + 1] * d1;
         ru += occ[k + 2] * d2;
         ru += occ[k + 3] * d3;
     }
-    for (; k < n_orb; ++k) ru += occ[k] * dk[(size_t)k * npt];
+    for (; k < n_orb; ++k) ru += occ[k] * dk[k * nd];
     if (rho_frozen) ru += 0.5 * rho_frozen[i] * omega[j];
     const double rho = ru / (2.0 * M_PI * omega[j] * wr[i] * r[i] * r[i]);
     const size_t o = (size_t)inst * N * Nth + idx;
@@ -2504,7 +2545,7 @@ static size_t footprint_bytes(const ggb_problem *p, int ntiles) {
     d += 2 * Nth * nh + 2 * Nth;         // PL, PLT, chi, omega
     d += 5 * N + p->n_orb;               // r, wr, v_gs, mask, rho_frozen, occ
     d += (2 * nh + 1) * N;               // Hartree ratio powers and 1/r
-    d += C * N * Nth;                    // dens
+    d += C * N * 2 * (size_t)ang_geometry(p).hq;  // dens (folded rows)
     d += 3 * nI * N * Nth;               // vxc, rho_u, dv
     d += 2 * nI * N * Nth;               // phase (double2)
     d += 2 * nI * nh * N;                // q, V
@@ -2614,7 +2655,7 @@ extern "C" int ggb_create(const ggb_problem *p, ggb_ctx **out) {
     ok &= dalloc(&ctx->occ, (size_t)p->n_orb) == hipSuccess;
     ok &= dalloc(&ctx->mcol, (size_t)p->n_orb) == hipSuccess;
     ok &= dalloc(&ctx->active, (size_t)p->n_inst) == hipSuccess;
-    ok &= dalloc(&ctx->dens, C * N * Nth) == hipSuccess;
+    ok &= dalloc(&ctx->dens, C * N * 2 * (size_t)ctx->ang.hq) == hipSuccess;  // folded |Psi|^2 rows
     ok &= dalloc(&ctx->vxc, nI * N * Nth) == hipSuccess;
     ok &= dalloc(&ctx->rho_u, nI * N * Nth) == hipSuccess;
     ok &= dalloc(&ctx->q, nI * (size_t)p->n_har * N) == hipSuccess;
@@ -2907,10 +2948,12 @@ static int launch_potential(ggb_ctx *ctx, const ObsArgs *ob = nullptr, MarkK mk 
         const dim3 gr(nrho + nobs, p.n_inst);
         if (p.precision)
             k_rho<float><<<gr, WG, 0, st>>>(reinterpret_cast<const float *>(ctx->dens), ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
-                                            ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb, oa, ctx->active);
+                                            ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, ctx->ang.h, ctx->ang.hq, p.n_orb, oa,
+                                            ctx->active);
         else
             k_rho<double><<<gr, WG, 0, st>>>(ctx->dens, ctx->occ, ctx->rho_frozen, ctx->r, ctx->wr,
-                                             ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, p.n_orb, oa, ctx->active);
+                                             ctx->omega, ctx->rho_u, ctx->vxc, N, Nth, ctx->ang.h, ctx->ang.hq, p.n_orb, oa,
+                                             ctx->active);
         GemmOperands gq;  // q[L][i] = sum_j PLT[L][j] rho_u[i][j]
         gq.A = ctx->PLT;
         gq.sam = Nth;
