@@ -422,11 +422,12 @@ def precision_report(sysm, pulse, n_batch=16):
         t0 = time.perf_counter()
         with Propagator(sysm, n_inst=ninst, precision=prec) as prop:
             prop.set_field(np.stack(fields[:ninst]) if ninst > 1 else E)
-            if ninst > 1:
-                prop.autotune(reps=3)
+            # the batch takes the autotuner's pick, as the batched leg does; the
+            # variants differ in their FP32 chunking, so the errors come with it
+            variant = prop.autotune(reps=3)[0] if ninst > 1 else "narrow"
             o = prop.run(0, n)
         runs[name] = (sysm.total_acceleration(o["acc"][:, 0, :], o["norm"][:, 0, :], E_end),
-                      o["dip"][:, 0, :] @ sysm.occ, time.perf_counter() - t0)
+                      o["dip"][:, 0, :] @ sysm.occ, time.perf_counter() - t0, variant)
         log("precision %s full run: %d steps, %d instance(s), %.1f s" % (name, n, ninst, runs[name][2]))
     a64, d64 = runs["fp64"][0], runs["fp64"][1]
     w, S64 = sp.acceleration_spectrum(a64, DT)
@@ -441,7 +442,7 @@ def precision_report(sysm, pulse, n_batch=16):
            "one_percent_in_log10": float(np.log10(1.01)), "wall_s_fp64": runs["fp64"][2],
            "batch_instances": n_batch}
     for name in ("fp32", "fp32_batched", "fp32_split_batched"):
-        a32, d32, wall = runs[name]
+        a32, d32, wall, variant = runs[name]
         _, S32 = sp.acceleration_spectrum(a32, DT)
         _, D32 = sp.dipole_spectrum(d32, DT)
         err = np.log10(sp.harmonic_intensities(w, S32, pulse.omega0, orders) / I64)
@@ -452,7 +453,7 @@ def precision_report(sysm, pulse, n_batch=16):
                      "dipole_form_log10_error": errd.tolist(),
                      "max_abs_log10_err_plateau": float(np.abs(err[plateau]).max()),
                      "max_abs_log10_err_cutoff": float(np.abs(err[~plateau]).max()),
-                     "wall_s": wall}
+                     "kinetic_variant": variant, "wall_s": wall}
     return rep
 
 
