@@ -2059,32 +2059,48 @@ struct QmomEpi {
 // wave by shuffles and across the four waves through LDS (one barrier), in a
 // fixed order; each thread then re-runs its chunk from its start value.
 // ---------------------------------------------------------------------------
-// inclusive scan of affine maps x -> al x + be over the work-group, maps of
-// lower threads applied first (dir = +1) or of higher threads first (dir = -1)
-static __device__ void affine_scan(double &al, double &be, int dir, double (*sm)[2]) {
+// inclusive scans of affine maps x -> a x + b over the work-group, both at
+// once: (al, be) with the maps of lower threads applied first, (sa, sb) with
+// those of higher threads first.  The two share each shuffle round and each
+// barrier, and each keeps the operation order of a scan run alone.
+static __device__ void affine_scan2(double &al, double &be, double &sa, double &sb, double (*sm)[4]) {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     for (int off = 1; off < 64; off <<= 1) {
-        const double pa = dir > 0 ? __shfl_up(al, off) : __shfl_down(al, off);
-        const double pb = dir > 0 ? __shfl_up(be, off) : __shfl_down(be, off);
-        if (dir > 0 ? lane >= off : lane + off < 64) {
+        const double pa = __shfl_up(al, off), pb = __shfl_up(be, off);
+        const double qa = __shfl_down(sa, off), qb = __shfl_down(sb, off);
+        if (lane >= off) {
             be = pb * al + be;
             al = pa * al;
         }
+        if (lane + off < 64) {
+            sb = qb * sa + sb;
+            sa = qa * sa;
+        }
     }
-    if (lane == (dir > 0 ? 63 : 0)) {
+    if (lane == 63) {
         sm[wave][0] = al;
         sm[wave][1] = be;
     }
+    if (lane == 0) {
+        sm[wave][2] = sa;
+        sm[wave][3] = sb;
+    }
     __syncthreads();
-    // maps of the whole waves before this one, nearest last
+    // maps of the whole waves before this one (forward) or after it (backward), nearest last
     double ca = 1.0, cb = 0.0;
-    for (int w = (dir > 0 ? 0 : WG / 64 - 1); dir > 0 ? w < wave : w > wave; w += dir) {
+    for (int w = 0; w < wave; ++w) {
         cb = cb * sm[w][0] + sm[w][1];
         ca = ca * sm[w][0];
     }
     be = cb * al + be;
     al = ca * al;
-    __syncthreads();
+    double da = 1.0, db = 0.0;
+    for (int w = WG / 64 - 1; w > wave; --w) {
+        db = db * sm[w][2] + sm[w][3];
+        da = da * sm[w][2];
+    }
+    sb = db * sa + sb;
+    sa = da * sa;
 }
 
 // hp: the ratio powers and 1/r, from the host (shared by all instances):
@@ -2098,7 +2114,7 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
                                                 const double *__restrict__ hp,
                                                 double *__restrict__ V, int N, int n_har,
                                                 const int32_t *__restrict__ active) {
-    __shared__ double sm[WG / 64][2];
+    __shared__ double sm[WG / 64][4];
     __shared__ double edge[2][WG];
     const int L = blockIdx.x, inst = blockIdx.y;
     if (active && !active[inst]) return;  // idle serving slot (block-uniform)
@@ -2132,8 +2148,6 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
             al *= ra[c];
             be = be * ra[c] + rq[c] * ri[c];
         }
-    affine_scan(al, be, 1, sm);
-    edge[0][t] = be;  // A at the end of this chunk
     // ---- suffix: B_a = gamma_a B_{a+1} + gamma_a q_{a+1}/r_{a+1}, B_{N-1} = 0
     double sa = 1.0, sb = 0.0;
 #pragma unroll
@@ -2150,7 +2164,8 @@ __global__ __launch_bounds__(WG) void k_hartree(const double *__restrict__ q,
             sa *= gam;
             sb = sb * gam + gam * q1 * i1;
         }
-    affine_scan(sa, sb, -1, sm);
+    affine_scan2(al, be, sa, sb, sm);
+    edge[0][t] = be;  // A at the end of this chunk
     edge[1][t] = sb;  // B at the start of this chunk
     __syncthreads();
     double A = t > 0 ? edge[0][t - 1] : 0.0;
