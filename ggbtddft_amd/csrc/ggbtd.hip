@@ -1904,6 +1904,10 @@ __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
     const double *Bc = g.B + z * g.sbz + (long)(nok ? n : 0) * g.sbn;
     const int kq = (((g.K + 3) / 4 + 3) / 4) * 4;  // quarter of K, multiple of 4
     const int kb = wave * kq, ke = min(g.K, kb + kq);
+    // wave w finishes accumulator row r = w: the epilogue's other inputs of
+    // that element are loaded ahead of the products (in-range indices)
+    const int mo = m0 + mfma_t<double>::row(kr, wave);
+    const typename Epi::Pre pre = epi.pre(z, mo < g.M ? mo : 0, nok ? n : 0);
     d4_t acc = {0.0, 0.0, 0.0, 0.0};
     // TG_KU loads of each operand in flight per lane; 9 covers a quarter of
     // K = 129 (N_theta or the multipole count at L_max = 64) in one round
@@ -1925,9 +1929,8 @@ __global__ __launch_bounds__(WG) void k_tile_gemm(GemmOperands g, Epi epi) {
     for (int r = 0; r < 4; ++r) red[wave > 0 ? wave - 1 : 3][lane][r] = acc[r];
     __syncthreads();
     const int r = wave;
-    const int mo = m0 + mfma_t<double>::row(kr, r);
     const double v = ((red[3][lane][r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
-    if (mo < g.M && nok) epi(z, mo, n, v);
+    if (mo < g.M && nok) epi.apply(z, mo, n, v, pre);
 }
 
 // The same products with one operand panel reused across tiles: a work-group
