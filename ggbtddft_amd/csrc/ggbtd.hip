@@ -297,6 +297,17 @@ __global__ __launch_bounds__(KIN_WG_MAX) void k_kin_fwd(const T *__restrict__ Q,
     const int kb = wave * kq, ke = min(N, kb + kq);
     double rc = 0.0;
     const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, false, &rc) : nullptr;
+    // wave w < 4 finishes accumulator row w of every tile: its phase factors
+    // are loaded ahead of the products
+    const int ar = mfma_t<T>::row(kr, wave & 3);
+    double2 kp[TA];
+    if (wave < 4) {
+#pragma unroll
+        for (int t = 0; t < TA; ++t) {
+            const int a = a0 + TA * ar + t;
+            kp[t] = kph[(size_t)l * nk_max + (a < nk ? a : 0)];
+        }
+    }
     v4 acc[TA];
 #pragma unroll
     for (int t = 0; t < TA; ++t) acc[t] = v4{0, 0, 0, 0};
@@ -331,14 +342,13 @@ __global__ __launch_bounds__(KIN_WG_MAX) void k_kin_fwd(const T *__restrict__ Q,
     __syncthreads();
     if (wave >= 4) return;
     const int r = wave;
-    const int ar = mfma_t<T>::row(kr, r);
 #pragma unroll
     for (int t = 0; t < TA; ++t) {
         double v = red[NW - 1][t][lane][r];
         for (int w = 0; w < NW - 1; ++w) v += red[w][t][lane][r];
         const double vo = lane_xor1(v);
         const int a = a0 + TA * ar + t;
-        const T o = kin_factor<T>(v, vo, p, kph[(size_t)l * nk_max + (a < nk ? a : 0)]);
+        const T o = kin_factor<T>(v, vo, p, kp[t]);
         if (colok && a < nk) Y[(((size_t)l * C + c) * 2 + p) * nk_max + a] = o;
     }
 }
@@ -391,6 +401,12 @@ __global__ __launch_bounds__(KIN_WG_MAX) void k_kin_bwd(const T *__restrict__ QT
     const T *Yr = Y + (((size_t)l * C + (colok ? c : 0)) * 2 + p) * nk_max + 4 * kr;
     const int kq = (((kend + 15) / 16 + NW - 1) / NW) * 16;  // K share of a wave, multiple of 16
     const int kb = wave * kq, ke = min(kend, kb + kq);
+    // wave w < 4 finishes accumulator row w: the mixed form's ground-state term
+    // of that element is loaded ahead of the products
+    const int io = i0 + mfma_t<T>::row(kr, wave & 3);
+    double rc = 0.0, rv = 0.0;
+    const double *rrow = (mfma_t<T>::mixed && colok && wave < 4) ? kin_ref_row(ref, l, c, p, N, n_orb, true, &rc) : nullptr;
+    if (rrow) rv = rrow[io < N ? io : 0];
     v4 acc = {0, 0, 0, 0};
     for (int k0 = kb; k0 < ke; k0 += 16 * KU2) {
         v4 av[KU2], bv[KU2];
@@ -416,15 +432,12 @@ __global__ __launch_bounds__(KIN_WG_MAX) void k_kin_bwd(const T *__restrict__ QT
     __syncthreads();
     if (wave >= 4) return;
     double *Xl = psi + (size_t)l * C * N * 2;
-    double rc = 0.0;
-    const double *rrow = (mfma_t<T>::mixed && colok) ? kin_ref_row(ref, l, c, p, N, n_orb, true, &rc) : nullptr;
     const int r = wave;
-    const int i = i0 + mfma_t<T>::row(kr, r);
     double v = red[NW - 1][lane][r];
     for (int w = 0; w < NW - 1; ++w) v += red[w][lane][r];
-    if (colok && i < N) {
-        if (rrow) v += rc * rrow[i];
-        Xl[((size_t)c * N + i) * 2 + p] = v;
+    if (colok && io < N) {
+        if (rrow) v += rc * rv;
+        Xl[((size_t)c * N + io) * 2 + p] = v;
     }
 }
 
