@@ -507,8 +507,8 @@ def main():
                           "GBps_read_plus_write": 2 * nb / (ms.value * 1e-3) / 1e9}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs single-device instances "
-              "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs single-device contexts "
+              "(sharding across GPUs is not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from ggbtddft_amd.engine import Propagator
